@@ -1,0 +1,212 @@
+/* sovereign_hip.h - C ABI of the MI355X (gfx950) linear-algebra core of the
+ * Sovereign LP solver's interior-point path.
+ *
+ * Plain C: pointers, sizes and int status codes; no C++ or torch types cross
+ * this boundary and nothing throws. Host pointers are named *_host (or are
+ * host pointers when not marked); device pointers are named *_dev.
+ *
+ * Status codes: 0 ok, 1 singular / numerical breakdown, < 0 errors.
+ *
+ * Each entry point names the reference interface it replaces
+ * (paths relative to the reference checkout).
+ */
+#ifndef SOVEREIGN_HIP_H
+#define SOVEREIGN_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SOV_OK 0
+#define SOV_SINGULAR 1
+#define SOV_ERR_ARG (-1)
+#define SOV_ERR_DEVICE (-2)
+#define SOV_ERR_NOMEM (-3)
+#define SOV_ERR_NODEV (-4)
+
+/* ---------------------------------------------------------------- device --
+ * Replaces cuda::DeviceInfo / cuda::device_info()
+ * (solver/gpu/include/sovereign/cuda_driver.hpp:14-27, probe at
+ * solver/gpu/src/cuda_driver.cpp:661-750). Probes once per process: opens
+ * device 0, checks the kernels on small problems (the self-test of
+ * cuda_driver.cpp:616-659) and reports why the GPU is unusable otherwise.
+ * SOVEREIGN_DISABLE_HIP=1 disables the device. */
+typedef struct {
+  int32_t available;
+  char name[256];
+  char arch[64];
+  int32_t compute_major;
+  int32_t compute_minor;
+  uint64_t memory_bytes;
+  int32_t driver_version;
+  int32_t multiprocessors;
+  char reason[256];
+} sov_dev_info;
+
+int sov_dev_probe(sov_dev_info* out);
+
+/* Per-process counters: gpu_operations / gpu_factorizations /
+ * reset_gpu_operations (solver/gpu/include/sovereign/gpu_spmv.hpp:89-92). */
+uint64_t sov_gpu_operations(void);
+uint64_t sov_gpu_factorizations(void);
+void sov_reset_gpu_operations(void);
+
+/* Last error text of the calling thread ("" when none). */
+const char* sov_last_error(void);
+
+/* Synchronises the library stream (all work is issued on it). */
+int sov_sync(void);
+/* The library's HIP stream (hipStream_t), for callers that time kernels. */
+void* sov_stream(void);
+
+/* Kernel timing: with timing on, the factorization and solve kernels are
+ * bracketed by HIP events on the library stream (this adds host waits).
+ * sov_kernel_times fills out[0..6] with the totals since the last call and
+ * resets them: trailing-update (SYRK) ms, SYRK launches, SYRK flops
+ * (algorithmic, lower triangle), panel (potrf + trsm) ms, triangular-solve
+ * ms, assembly ms, panel launches. No counterpart in the reference, which
+ * times nothing on the device. */
+void sov_set_kernel_timing(int32_t on);
+int sov_kernel_times(double* out7);
+
+/* -------------------------------------------------------------- dense LU --
+ * Replaces cuda::dense_lu (cuda_driver.hpp:29-34; run_dense_lu at
+ * cuda_driver.cpp:461-502). In-place LU with partial pivoting of an n x n
+ * column-major host matrix: unit-lower L below the diagonal, U on and above
+ * it, piv the row permutation in DenseLU's convention (row i of P*A is row
+ * piv[i] of A; solver/numerical/src/dense_lu.cpp:19-40). Singular (1) when
+ * the largest remaining pivot magnitude is below 1e-14. */
+int sov_dense_lu(int64_t n, double* a_host, int32_t* piv_host);
+
+/* ------------------------------------------------------------------ SpMV --
+ * Replaces cuda::spmv_csc (cuda_driver.hpp:36-40) and its content-compared
+ * per-thread MatrixCache (cuda_driver.cpp:509-582) by an explicit handle:
+ * the matrix is uploaded once (CSR of A for y = A x, the CSC arrays as CSR
+ * of A^T for y = A^T x) and stays resident in HBM. int32 indices, FP64. */
+typedef struct sov_mat sov_mat;
+
+int sov_mat_upload_csc(int32_t nrows, int32_t ncols, const int32_t* col_ptr,
+                       const int32_t* row_idx, const double* values, sov_mat** out);
+int sov_mat_free(sov_mat* a);
+/* y = A x (transpose = 0) or y = A^T x (transpose = 1), host vectors. */
+int sov_spmv(sov_mat* a, int32_t transpose, const double* x_host, double* y_host);
+/* Same on device vectors, asynchronous on the library stream. */
+int sov_spmv_dev(sov_mat* a, int32_t transpose, const double* x_dev, double* y_dev);
+
+/* One-shot form of cuda::spmv_csc: keeps one resident copy per calling
+ * thread and direction, replaced when the matrix contents change. */
+int sov_spmv_csc(int32_t nrows, int32_t ncols, const int32_t* col_ptr, const int32_t* row_idx,
+                 const double* values, const double* x_host, double* y_host, int32_t transpose);
+
+/* ------------------------------------------------------- normal equations --
+ * The IPM's KKT-solve interface: NormalEquations{factor(d); solve(rhs)}
+ * (solver/lp/src/interior_point.cpp:129-212). M = A diag(d) A^T + 1e-12 I is
+ * assembled in HBM and factored by a blocked dense Cholesky on MFMA. Pivots
+ * follow the reference's sparse LDL^T (solver/numerical/src/sparse_ldlt.cpp:
+ * 165-178): a pivot <= 1e-13 x |original diagonal| drops that direction (the
+ * reference's 1e128 replacement), a non-finite or < 1e-30 pivot fails.
+ * precision: 0 FP64 factor; 1 FP32 factor (FP32 MFMA, FP32 accumulation) with
+ * FP64 preconditioned-CG refinement against the FP64 matrix-free product. */
+typedef struct sov_ne sov_ne;
+
+#define SOV_PREC_FP64 0
+#define SOV_PREC_FP32 1
+
+int sov_ne_create(sov_mat* a, sov_ne** out);
+/* factor(d): 0 ok, 1 breakdown. d_host has ncols entries. */
+int sov_ne_factor(sov_ne* h, const double* d_host, int32_t precision);
+/* solve(rhs) in place, nrows entries. */
+int sov_ne_solve(sov_ne* h, double* rhs_host);
+/* Relative residual ||rhs - M x|| / ||rhs|| of the last solve (inf-norms). */
+double sov_ne_last_residual(const sov_ne* h);
+int32_t sov_ne_replaced_pivots(const sov_ne* h);
+int sov_ne_free(sov_ne* h);
+
+/* ----------------------------------------------------------------- IPM ----
+ * InteriorPointSolver (solver/lp/include/sovereign/interior_point.hpp:7-36;
+ * Mehrotra loop solve_ipm at solver/lp/src/interior_point.cpp:454-740) with
+ * every iteration resident on the GPU. */
+typedef struct {
+  int32_t max_iterations;      /* 250 */
+  double feasibility_tol;      /* 1e-9 */
+  double optimality_tol;       /* 1e-9 */
+  double fraction_to_boundary; /* 0.999 */
+  int32_t enable_scaling;      /* 1 (general-form solve only) */
+  /* SOV_PREC_FP64: every factorization in FP64 (the plain FP64 GPU path).
+   * SOV_PREC_FP32: mixed precision - FP32 factorizations while
+   * mu >= mixed_switch_mu, FP64 afterwards; FP64 residuals and refinement. */
+  int32_t precision;
+  double mixed_switch_mu; /* 1e-6 */
+  int32_t refine_max;     /* 30: PCG iterations per solve at most */
+  double refine_tol;      /* 1e-12: relative residual target of a solve */
+} sov_ipm_options;
+
+typedef struct {
+  int32_t status; /* sovereign::SolverStatus: 0 Optimal, 1 Feasible, 6 NumericalError, 7 Error */
+  int32_t iterations;
+  double objective;
+  double duality_gap;
+  double primal_residual;
+  double dual_residual;
+  int32_t factorizations;
+  int32_t fp32_factorizations;
+  int32_t refine_iterations; /* PCG iterations summed over all solves */
+  double max_solve_residual; /* worst relative residual of a Newton solve */
+  double seconds;            /* whole solve, device-synchronised */
+  double factor_seconds;     /* assembly + Cholesky, from HIP events */
+  double solve_seconds;      /* triangular solves + refinement, from HIP events */
+} sov_ipm_result;
+
+void sov_ipm_default_options(sov_ipm_options* opt);
+
+/* Resident solver for one constraint matrix (the IPM form's A, m x n):
+ * create analyses A once (the normal-equation assembly plan, the role of the
+ * NormalEquations constructor, interior_point.cpp:131-161), solve runs the
+ * Mehrotra loop for a right-hand side b (m) and cost c (n). The matrix handle
+ * must outlive the solver. */
+typedef struct sov_ipm sov_ipm;
+int sov_ipm_create(sov_mat* a, sov_ipm** out);
+int sov_ipm_solve(sov_ipm* h, const double* b, const double* c, const sov_ipm_options* opt,
+                  sov_ipm_result* res, double* x, double* y, double* s);
+int sov_ipm_free(sov_ipm* h);
+
+/* LP already in IPM form: min c'x, A x = b, x >= 0 (A is m x n CSC). x, s
+ * (n) and y (m) are host outputs and may be NULL. The objective is c'x. */
+int sov_ipm_solve_form(int32_t m, int32_t n, const int32_t* col_ptr, const int32_t* row_idx,
+                       const double* values, const double* b, const double* c,
+                       const sov_ipm_options* opt, sov_ipm_result* res, double* x, double* y,
+                       double* s);
+
+/* General LP (the array image of sovereign::OptimizationModel for LP,
+ * solver/core/include/sovereign/types.hpp:32-59): CSC constraint matrix
+ * n_rows x n_vars, row senses 'L' / 'G' / 'E', bounds (ub >= 1e29 = none). */
+typedef struct {
+  int32_t n_vars;
+  int32_t n_rows;
+  const int32_t* col_ptr;
+  const int32_t* row_idx;
+  const double* values;
+  const double* obj;
+  double obj_constant;
+  int32_t maximize;
+  const double* lb;
+  const double* ub;
+  const char* row_sense;
+  const double* rhs;
+} sov_lp_model;
+
+/* InteriorPointSolver::solve (interior_point.cpp:747-757): build_ipm_form
+ * (:224-409) on the host, the Mehrotra loop on the GPU, then the result
+ * mapped back (:541-562). primal / var_dual: n_vars, row_dual: n_rows; any
+ * may be NULL. */
+int sov_lp_solve(const sov_lp_model* model, const sov_ipm_options* opt, sov_ipm_result* res,
+                 double* primal, double* row_dual, double* var_dual);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SOVEREIGN_HIP_H */

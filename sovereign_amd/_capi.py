@@ -1,0 +1,172 @@
+"""ctypes binding of libsovereign_hip.so (include/sovereign_hip.h).
+
+The library is built in-tree by `make -C sovereign_amd` (or
+``__graft_entry__.build()``). Loading fails loudly when it is missing: there is
+no CPU fallback behind this module.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libsovereign_hip.so")
+
+SOV_OK = 0
+SOV_SINGULAR = 1
+SOV_PREC_FP64 = 0
+SOV_PREC_FP32 = 1
+
+# Declared C entry points (name -> (restype, argtypes)); the CPU test suite
+# checks that the built library exports every one of them.
+_i32p = C.POINTER(C.c_int32)
+_f64p = C.POINTER(C.c_double)
+_vp = C.c_void_p
+
+
+class DevInfo(C.Structure):
+    _fields_ = [
+        ("available", C.c_int32),
+        ("name", C.c_char * 256),
+        ("arch", C.c_char * 64),
+        ("compute_major", C.c_int32),
+        ("compute_minor", C.c_int32),
+        ("memory_bytes", C.c_uint64),
+        ("driver_version", C.c_int32),
+        ("multiprocessors", C.c_int32),
+        ("reason", C.c_char * 256),
+    ]
+
+
+class IpmOptions(C.Structure):
+    _fields_ = [
+        ("max_iterations", C.c_int32),
+        ("feasibility_tol", C.c_double),
+        ("optimality_tol", C.c_double),
+        ("fraction_to_boundary", C.c_double),
+        ("enable_scaling", C.c_int32),
+        ("precision", C.c_int32),
+        ("mixed_switch_mu", C.c_double),
+        ("refine_max", C.c_int32),
+        ("refine_tol", C.c_double),
+    ]
+
+
+class IpmResult(C.Structure):
+    _fields_ = [
+        ("status", C.c_int32),
+        ("iterations", C.c_int32),
+        ("objective", C.c_double),
+        ("duality_gap", C.c_double),
+        ("primal_residual", C.c_double),
+        ("dual_residual", C.c_double),
+        ("factorizations", C.c_int32),
+        ("fp32_factorizations", C.c_int32),
+        ("refine_iterations", C.c_int32),
+        ("max_solve_residual", C.c_double),
+        ("seconds", C.c_double),
+        ("factor_seconds", C.c_double),
+        ("solve_seconds", C.c_double),
+    ]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+class LpModelC(C.Structure):
+    _fields_ = [
+        ("n_vars", C.c_int32),
+        ("n_rows", C.c_int32),
+        ("col_ptr", _i32p),
+        ("row_idx", _i32p),
+        ("values", _f64p),
+        ("obj", _f64p),
+        ("obj_constant", C.c_double),
+        ("maximize", C.c_int32),
+        ("lb", _f64p),
+        ("ub", _f64p),
+        ("row_sense", C.c_char_p),
+        ("rhs", _f64p),
+    ]
+
+
+SIGNATURES = {
+    "sov_dev_probe": (C.c_int, [C.POINTER(DevInfo)]),
+    "sov_gpu_operations": (C.c_uint64, []),
+    "sov_gpu_factorizations": (C.c_uint64, []),
+    "sov_reset_gpu_operations": (None, []),
+    "sov_last_error": (C.c_char_p, []),
+    "sov_sync": (C.c_int, []),
+    "sov_stream": (_vp, []),
+    "sov_set_kernel_timing": (None, [C.c_int32]),
+    "sov_kernel_times": (C.c_int, [_f64p]),
+    "sov_dense_lu": (C.c_int, [C.c_int64, _f64p, _i32p]),
+    "sov_mat_upload_csc": (C.c_int, [C.c_int32, C.c_int32, _i32p, _i32p, _f64p, C.POINTER(_vp)]),
+    "sov_mat_free": (C.c_int, [_vp]),
+    "sov_spmv": (C.c_int, [_vp, C.c_int32, _f64p, _f64p]),
+    "sov_spmv_dev": (C.c_int, [_vp, C.c_int32, _vp, _vp]),
+    "sov_spmv_csc": (C.c_int, [C.c_int32, C.c_int32, _i32p, _i32p, _f64p, _f64p, _f64p, C.c_int32]),
+    "sov_ne_create": (C.c_int, [_vp, C.POINTER(_vp)]),
+    "sov_ne_factor": (C.c_int, [_vp, _f64p, C.c_int32]),
+    "sov_ne_solve": (C.c_int, [_vp, _f64p]),
+    "sov_ne_last_residual": (C.c_double, [_vp]),
+    "sov_ne_replaced_pivots": (C.c_int32, [_vp]),
+    "sov_ne_free": (C.c_int, [_vp]),
+    "sov_ipm_default_options": (None, [C.POINTER(IpmOptions)]),
+    "sov_ipm_create": (C.c_int, [_vp, C.POINTER(_vp)]),
+    "sov_ipm_solve": (C.c_int, [_vp, _f64p, _f64p, C.POINTER(IpmOptions), C.POINTER(IpmResult),
+                                _f64p, _f64p, _f64p]),
+    "sov_ipm_free": (C.c_int, [_vp]),
+    "sov_ipm_solve_form": (C.c_int, [C.c_int32, C.c_int32, _i32p, _i32p, _f64p, _f64p, _f64p,
+                                     C.POINTER(IpmOptions), C.POINTER(IpmResult), _f64p, _f64p, _f64p]),
+    "sov_lp_solve": (C.c_int, [C.POINTER(LpModelC), C.POINTER(IpmOptions), C.POINTER(IpmResult),
+                               _f64p, _f64p, _f64p]),
+}
+
+_lib = None
+
+
+def load() -> C.CDLL:
+    """Loads the in-tree library (raises when it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(
+            f"{LIB_PATH} is missing: build it with `make -C sovereign_amd` "
+            "or __graft_entry__.build(); there is no CPU fallback.")
+    lib = C.CDLL(LIB_PATH)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def f64(a) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def i32(a) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+def pf(a: np.ndarray):
+    return a.ctypes.data_as(_f64p) if a is not None else None
+
+
+def pi(a: np.ndarray):
+    return a.ctypes.data_as(_i32p) if a is not None else None
+
+
+def last_error() -> str:
+    return load().sov_last_error().decode()
+
+
+def check(rc: int, what: str) -> int:
+    if rc < 0:
+        raise RuntimeError(f"{what} failed ({rc}): {last_error()}")
+    return rc

@@ -1,0 +1,65 @@
+// Blocked dense Cholesky of the normal equations and its triangular solves.
+#pragma once
+
+#include "common.hpp"
+
+namespace sov {
+
+// Replacement value for a dropped pivot: L(k,k) = kDropPivot<T>, so the
+// direction is scaled out (the reference's LDL^T stores d_k = 1e128).
+template <typename T>
+struct DropPivot;
+template <>
+struct DropPivot<double> {
+  static constexpr double value = 1e64;
+};
+template <>
+struct DropPivot<float> {
+  static constexpr float value = 1e18f;
+};
+
+constexpr int kPanel = 128;  // factorization panel width (K of the trailing update)
+constexpr int kSolveBlock = 64;  // row block of the triangular solves
+
+// Per-factorization device state shared by the kernels.
+struct FactorFlags {
+  int status;    // 0 ok, 1 breakdown (non-finite or < 1e-30 pivot)
+  int replaced;  // dropped pivots
+  unsigned ticket;  // persistent triangular-solve work counter (monotone)
+  unsigned pad;
+};
+
+// Owns the factor workspace for an m x m matrix stored column-major in `a`
+// (lower triangle used). All work goes to `s`.
+template <typename T>
+class DenseCholesky {
+ public:
+  ~DenseCholesky();
+  int reserve(int m);  // allocates dinv, flags, solve flags
+  // In-place factorization of a (lower); diag0 holds the diagonal before
+  // elimination (for the drop test). Asynchronous; status in flags().
+  int factor(T* a, int m, const double* diag0, hipStream_t s);
+  // x <- L^-T L^-1 x on device vector x (FP64), using the factor in a.
+  int solve(const T* a, int m, double* x, double* work, hipStream_t s);
+  FactorFlags* flags() const { return flags_; }
+
+ private:
+  int m_ = 0;
+  T* dinv_ = nullptr;         // inverses of the kSolveBlock diagonal blocks
+  unsigned* solve_flags_ = nullptr;  // per row block: generation it was published in
+  FactorFlags* flags_ = nullptr;
+  unsigned generation_ = 0;   // solve launches so far (each publishes all blocks once)
+  unsigned tickets_used_ = 0;
+};
+
+// Global switch for per-kernel event timing (adds synchronisation) and the
+// totals it accumulates (sov_kernel_times).
+struct KernelTimes {
+  double syrk_ms = 0.0, syrk_launches = 0.0, syrk_flops = 0.0;
+  double panel_ms = 0.0, solve_ms = 0.0, assemble_ms = 0.0, panel_launches = 0.0;
+};
+void set_kernel_timing(bool on);
+bool kernel_timing();
+KernelTimes& kernel_times();
+
+}  // namespace sov

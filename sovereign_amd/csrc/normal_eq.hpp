@@ -1,0 +1,74 @@
+// Normal equations M = A diag(d) A^T + reg I on the device: assembly plan,
+// factorization (FP64, or FP32 with FP64 preconditioned CG) and solves.
+#pragma once
+
+#include "chol.hpp"
+#include "internal.hpp"
+
+namespace sov {
+
+constexpr double kNormalReg = 1e-12;  // interior_point.cpp:99,103
+
+// Scalars of the preconditioned CG kept on the device.
+struct PcgScalars {
+  double rz, pq, alpha, beta, rnorm, bnorm;
+};
+
+class NormalSolver {
+ public:
+  ~NormalSolver();
+  int init(sov_mat* a);
+  // factor(d) with d on the device. Asynchronous; check breakdown().
+  int factor(const double* d_dev, int precision, hipStream_t s);
+  // Waits for the last factorization and reports 0 ok, 1 breakdown.
+  int breakdown(hipStream_t s);
+  // x <- M^-1 x in place (device vector of nrows). For an FP32 factor this
+  // runs PCG against the FP64 matrix-free product until the relative
+  // residual is below tol or max_iter is reached. Returns SOV_OK, or
+  // SOV_SINGULAR when PCG did not reach tol.
+  int solve(double* x_dev, hipStream_t s, double tol, int max_iter);
+
+  int precision() const { return prec_; }
+  int last_pcg_iterations() const { return last_iters_; }
+  double last_residual() const { return last_res_; }
+  int replaced_pivots() const { return replaced_; }
+  int dense_mode() const { return dense_a_ != nullptr; }
+  DenseCholesky<double> ch64;
+  DenseCholesky<float> ch32;
+  double assemble_ms = 0.0;
+
+ private:
+  template <typename T>
+  int assemble(const double* d_dev, T* M, hipStream_t s);
+  int matvec(const double* p, double* q, hipStream_t s);  // q = M p in FP64, matrix-free
+
+  sov_mat* A_ = nullptr;
+  int m_ = 0, n_ = 0;
+  const double* d_ = nullptr;  // d of the current factorization (device, caller-owned)
+  // Sparse assembly plan: entry e (a lower-triangle position of M) sums
+  // val[ta[t]] * d[tj[t]] * val[tb[t]] over t in [eptr[e], eptr[e+1]).
+  long long entries_ = 0, triples_ = 0;
+  int32_t* eptr_ = nullptr;
+  int32_t* ta_ = nullptr;
+  int32_t* tb_ = nullptr;
+  int32_t* tj_ = nullptr;
+  long long* eoff_ = nullptr;
+  int32_t* ediag_ = nullptr;  // row index for diagonal entries, -1 otherwise
+  // Dense-A plan (when the sparse plan would exceed the dense matrix):
+  double* dense_a_ = nullptr;  // m x n column-major
+  void* wbuf_ = nullptr;       // A diag(sqrt d), T = factor precision
+  double* M64_ = nullptr;
+  float* M32_ = nullptr;
+  double* diag0_ = nullptr;
+  // PCG workspace
+  double *r_ = nullptr, *z_ = nullptr, *p_ = nullptr, *q_ = nullptr, *t_ = nullptr, *b_ = nullptr;
+  double* partial_ = nullptr;
+  PcgScalars* sc_ = nullptr;
+  PcgScalars* sc_host_ = nullptr;
+  int prec_ = 0;
+  int last_iters_ = 0;
+  double last_res_ = 0.0;
+  int replaced_ = 0;
+};
+
+}  // namespace sov
