@@ -1,0 +1,89 @@
+/* sovereign_bnb.h - strong branching and branch-and-bound over batched child
+ * LPs (C ABI). Builds on sovereign_hip.h. Status codes as there. */
+#ifndef SOVEREIGN_BNB_H
+#define SOVEREIGN_BNB_H
+
+#include "sovereign_hip.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SOV_CHILD_BATCHED 0 /* all child LPs of a node in one persistent launch */
+#define SOV_CHILD_PER_LP 1  /* each child through the plain FP64 single-LP GPU path */
+
+/* One strong-branching candidate: the down (ub = floor x) and up
+ * (lb = ceil x) child LPs and the score of pick_strong_branch
+ * (solver/milp/src/branch_and_bound.cpp:726-793): degradation
+ * |child objective - parent objective|, 1e6 for an infeasible child, score
+ * min(down, up) + 1e-4 max(down, up). Objectives are in the model's sense;
+ * a child that did not reach optimality reports the parent objective (as
+ * the reference does for a child without an objective value). */
+typedef struct {
+  int32_t var;
+  int32_t status_down; /* 0 optimal, 2 infeasible (empty box), 5/6 unresolved */
+  int32_t status_up;
+  int32_t iters_down;
+  int32_t iters_up;
+  int32_t pad;
+  double obj_down;
+  double obj_up;
+  double score;
+} sov_sb_child;
+
+/* Strong branching at one node: the node LP is `node` with variable bounds
+ * node_lb / node_ub, x_node its LP solution and parent_obj its objective.
+ * Candidates are the fractional integer variables (is_int) ordered by
+ * closeness of their fraction to 1/2, at most max_candidates of them
+ * (the FullStrong rule, branch_and_bound.cpp:731-748). out holds one entry
+ * per candidate (max_candidates entries), ncand_out their number, best the
+ * index of the first best score (-1 when no candidate is fractional).
+ * kernel_ms accumulates the batched launch duration (may be NULL). */
+int sov_strong_branch(const sov_lp_model* node, const double* node_lb, const double* node_ub,
+                      const int8_t* is_int, const double* x_node, double parent_obj,
+                      int32_t max_candidates, int32_t child_solver, const sov_ipm_options* opt,
+                      sov_sb_child* out, int32_t* ncand_out, int32_t* best, double* kernel_ms);
+
+/* Ranking callback: fills key[j] for the variables of a node (higher =
+ * branch first); used to put the learned surrogate's order in front of the
+ * strong-branching candidate cut. */
+typedef void (*sov_candidate_order_fn)(int32_t n_vars, const double* x, const double* lb,
+                                       const double* ub, double* key, void* user);
+
+typedef struct {
+  int64_t max_nodes;
+  int32_t strong_branch_candidates; /* k candidates strong-branched per node */
+  int32_t child_solver;             /* SOV_CHILD_BATCHED or SOV_CHILD_PER_LP */
+  double mip_gap;                   /* 1e-6 relative (branch_and_bound.hpp:23) */
+  double integer_tol;               /* 1e-6 */
+  double time_limit_seconds;        /* 0 = none */
+  sov_candidate_order_fn candidate_order; /* NULL: most fractional first */
+  void* candidate_order_user;
+} sov_bnb_options;
+
+typedef struct {
+  int32_t status; /* 0 optimal, 1 feasible (limit hit), 2 infeasible, 5 limit without incumbent, 6 numerical */
+  int32_t pad;
+  double objective;
+  double root_bound;
+  int64_t nodes;
+  int64_t lp_solves;
+  double seconds;
+  double strong_branch_seconds;
+  double kernel_ms;
+} sov_bnb_result;
+
+void sov_bnb_default_options(sov_bnb_options* opt);
+
+/* Best-bound branch-and-bound with strong branching
+ * (BranchAndBoundSolver::solve, branch_and_bound.cpp:906-1523, without cuts,
+ * heuristics or node presolve). is_int marks integer variables; x_best
+ * (n_vars, may be NULL) receives the incumbent. */
+int sov_bnb_solve(const sov_lp_model* milp, const int8_t* is_int, const sov_bnb_options* bopt,
+                  const sov_ipm_options* opt, sov_bnb_result* res, double* x_best);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SOVEREIGN_BNB_H */

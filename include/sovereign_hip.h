@@ -205,6 +205,38 @@ typedef struct {
 int sov_lp_solve(const sov_lp_model* model, const sov_ipm_options* opt, sov_ipm_result* res,
                  double* primal, double* row_dual, double* var_dual);
 
+/* ------------------------------------------------------ batched child LPs --
+ * The data-parallel unit of strong branching: the 2k child LP relaxations of
+ * a branch-and-bound node (solve_two_lps_parallel ×k in pick_strong_branch /
+ * pick_reliability_branch, solver/milp/src/branch_and_bound.cpp:554-638,
+ * 726-899), solved together in one persistent launch. Each LP is
+ *     min c'x  s.t.  A x = b,  0 <= x <= u
+ * with a dense column-major A of at most 272 rows (u >= 1e29: no upper bound,
+ * u == 0: x fixed at 0). Matrices are uploaded once and shared by LPs;
+ * per-LP b lives at b_all + b_off[k] (m entries), c / u / x at
+ * + x_off[k] (n entries). status: 0 optimal, 5 iteration limit,
+ * 6 numerical breakdown. The objective is c'x of the given data. */
+typedef struct sov_batch sov_batch;
+typedef struct {
+  int32_t status;
+  int32_t iterations;
+  int32_t pcg_iterations;
+  int32_t pad;
+  double objective;
+  double duality_gap;
+  double primal_residual;
+  double dual_residual;
+} sov_batch_result;
+
+int sov_batch_create(sov_batch** out);
+int sov_batch_add_matrix(sov_batch* h, int32_t m, int32_t n, const double* a_colmajor, int32_t* id);
+int sov_batch_solve(sov_batch* h, int32_t count, const int32_t* matrix_id, const double* b_all,
+                    const int64_t* b_off, const double* c_all, const double* u_all, const int64_t* x_off,
+                    const sov_ipm_options* opt, sov_batch_result* res, double* x_all);
+/* Duration of the last solve's kernel (HIP events on the library stream). */
+double sov_batch_last_kernel_ms(const sov_batch* h);
+int sov_batch_free(sov_batch* h);
+
 #ifdef __cplusplus
 }
 #endif

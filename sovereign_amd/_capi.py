@@ -75,6 +75,50 @@ class IpmResult(C.Structure):
         return {name: getattr(self, name) for name, _ in self._fields_}
 
 
+class BatchResult(C.Structure):
+    _fields_ = [
+        ("status", C.c_int32),
+        ("iterations", C.c_int32),
+        ("pcg_iterations", C.c_int32),
+        ("pad", C.c_int32),
+        ("objective", C.c_double),
+        ("duality_gap", C.c_double),
+        ("primal_residual", C.c_double),
+        ("dual_residual", C.c_double),
+    ]
+
+
+class SbChild(C.Structure):
+    _fields_ = [
+        ("var", C.c_int32), ("status_down", C.c_int32), ("status_up", C.c_int32),
+        ("iters_down", C.c_int32), ("iters_up", C.c_int32), ("pad", C.c_int32),
+        ("obj_down", C.c_double), ("obj_up", C.c_double), ("score", C.c_double),
+    ]
+
+
+CANDIDATE_ORDER_FN = C.CFUNCTYPE(None, C.c_int32, C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                 C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_void_p)
+
+
+class BnbOptions(C.Structure):
+    _fields_ = [
+        ("max_nodes", C.c_int64), ("strong_branch_candidates", C.c_int32), ("child_solver", C.c_int32),
+        ("mip_gap", C.c_double), ("integer_tol", C.c_double), ("time_limit_seconds", C.c_double),
+        ("candidate_order", CANDIDATE_ORDER_FN), ("candidate_order_user", C.c_void_p),
+    ]
+
+
+class BnbResult(C.Structure):
+    _fields_ = [
+        ("status", C.c_int32), ("pad", C.c_int32), ("objective", C.c_double), ("root_bound", C.c_double),
+        ("nodes", C.c_int64), ("lp_solves", C.c_int64), ("seconds", C.c_double),
+        ("strong_branch_seconds", C.c_double), ("kernel_ms", C.c_double),
+    ]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_ if name != "pad"}
+
+
 class LpModelC(C.Structure):
     _fields_ = [
         ("n_vars", C.c_int32),
@@ -123,6 +167,18 @@ SIGNATURES = {
                                      C.POINTER(IpmOptions), C.POINTER(IpmResult), _f64p, _f64p, _f64p]),
     "sov_lp_solve": (C.c_int, [C.POINTER(LpModelC), C.POINTER(IpmOptions), C.POINTER(IpmResult),
                                _f64p, _f64p, _f64p]),
+    "sov_batch_create": (C.c_int, [C.POINTER(_vp)]),
+    "sov_batch_add_matrix": (C.c_int, [_vp, C.c_int32, C.c_int32, _f64p, _i32p]),
+    "sov_batch_solve": (C.c_int, [_vp, C.c_int32, _i32p, _f64p, C.POINTER(C.c_int64), _f64p, _f64p,
+                                  C.POINTER(C.c_int64), C.POINTER(IpmOptions), C.POINTER(BatchResult), _f64p]),
+    "sov_batch_last_kernel_ms": (C.c_double, [_vp]),
+    "sov_batch_free": (C.c_int, [_vp]),
+    "sov_strong_branch": (C.c_int, [C.POINTER(LpModelC), _f64p, _f64p, C.POINTER(C.c_int8), _f64p, C.c_double,
+                                    C.c_int32, C.c_int32, C.POINTER(IpmOptions), C.POINTER(SbChild), _i32p, _i32p,
+                                    _f64p]),
+    "sov_bnb_default_options": (None, [C.POINTER(BnbOptions)]),
+    "sov_bnb_solve": (C.c_int, [C.POINTER(LpModelC), C.POINTER(C.c_int8), C.POINTER(BnbOptions),
+                                C.POINTER(IpmOptions), C.POINTER(BnbResult), _f64p]),
 }
 
 _lib = None

@@ -1,0 +1,721 @@
+// Batched interior-point solves of many small dense LPs in one persistent
+// launch - the strong-branching children of a branch-and-bound node.
+//
+// The reference solves the 2k child LPs of a node one after another with a
+// warm-started dual simplex (pick_strong_branch / pick_reliability_branch,
+// solver/milp/src/branch_and_bound.cpp:726-899; solve_two_lps_parallel
+// :554-638 is serial on Linux). Here every child LP
+//
+//     min c'x   s.t.  A x = b,  0 <= x <= u   (u_j >= 1e29: no upper bound,
+//                                            u_j == 0: x_j fixed at 0)
+//
+// is solved by a bounded-variable Mehrotra predictor-corrector method run
+// entirely by one 1024-thread workgroup; workgroups take LPs from a queue, so
+// LPs of uneven size share one launch. Per iteration:
+//   M = A D A^T          FP32 MFMA (v_mfma_f32_16x16x4f32), FP32 accumulation,
+//                        written to LDS as a packed lower triangle (m <= 272);
+//   M = L L^T            FP32, in LDS, with the LDL^T pivot rule of the
+//                        reference (drop a pivot <= 1e-13 |diagonal|);
+//   M dy = rhs           FP64 preconditioned CG with the FP32 factor as the
+//                        preconditioner and the FP64 product A (D (A^T p)).
+// Residuals, step lengths and the termination test are FP64.
+
+#include "internal.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+typedef float flt4_t __attribute__((ext_vector_type(4)));
+
+namespace {
+
+constexpr int kThreads = 1024;
+constexpr int kWaves = kThreads / 64;
+constexpr int kMaxRows = 272;
+constexpr int kStage = 1024;  // doubles of LDS staging (vectors, partial sums)
+
+struct BatchMatrix {
+  int m, n;
+  const double* a64;  // column-major m x n
+  const float* a32;
+};
+
+struct BatchDesc {
+  int mat;
+  int pad;
+  long long b_off, x_off, ws_off;
+};
+
+struct BatchOpts {
+  double tol, tau, pcg_tol;
+  int max_iter, pcg_max;
+};
+
+}  // namespace
+
+typedef sov_batch_result sov_batch_result_dev;
+
+namespace {
+
+__device__ __forceinline__ long long packed_index(int i, int j, int m) {
+  // column-major packed lower triangle, i >= j
+  return static_cast<long long>(j) * (2 * m - j + 1) / 2 + (i - j);
+}
+
+// Block-wide reduction of up to 4 values; every thread gets the results.
+template <int OP>
+__device__ __forceinline__ double bred(double v, double* red) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  v = OP == 0 ? wave_sum(v) : OP == 1 ? wave_max(v) : wave_min(v);
+  __syncthreads();
+  if (lane == 0) red[w] = v;
+  __syncthreads();
+  double r = red[0];
+  for (int k = 1; k < kWaves; ++k) r = OP == 0 ? r + red[k] : OP == 1 ? fmax(r, red[k]) : fmin(r, red[k]);
+  return r;
+}
+
+struct Lp {
+  int m, n;
+  const double* A;
+  const float* Af;
+  const double *b, *c, *u;
+  // workspace vectors (global, this LP only)
+  double *x, *w, *z, *v, *rc, *ru, *D, *dx, *dw, *dz, *dv, *dxa, *dwa, *dza, *dva, *t;
+  double *y, *rb, *dy, *r, *p, *q, *zz, *rhs, *diag0;
+};
+
+// out_j = (A^T vec)_j for j < n; vec (m) is staged in LDS first.
+__device__ void gemv_t(const Lp& L, const double* vec, double* out, double* stage) {
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  __syncthreads();
+  for (int i = tid; i < L.m; i += kThreads) stage[i] = vec[i];
+  __syncthreads();
+  for (int j0 = wv * 4; j0 < L.n; j0 += kWaves * 4) {
+    double s[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int j = j0 + q;
+      if (j < L.n) {
+        const double* col = L.A + static_cast<long long>(j) * L.m;
+        for (int i = lane; i < L.m; i += 64) s[q] = fma(col[i], stage[i], s[q]);
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const double tot = wave_sum(s[q]);
+      if (lane == 0 && j0 + q < L.n) out[j0 + q] = tot;
+    }
+  }
+  __syncthreads();
+}
+
+// out_i = base_i + (A vec)_i for i < m (base may be null). vec (n) global.
+__device__ void gemv_n(const Lp& L, const double* vec, const double* base, double* out, double* stage) {
+  const int tid = threadIdx.x;
+  const int row = tid & 255, h = tid >> 8;  // 4 slices of the columns
+  for (int r0 = 0; r0 < L.m; r0 += 256) {
+    const int i = r0 + row;
+    double acc = 0.0;
+    if (i < L.m)
+      for (int j = h; j < L.n; j += 4) acc = fma(L.A[i + static_cast<long long>(j) * L.m], vec[j], acc);
+    __syncthreads();
+    stage[h * 256 + row] = acc;
+    __syncthreads();
+    if (h == 0 && i < L.m) {
+      const double s = stage[row] + stage[256 + row] + stage[512 + row] + stage[768 + row];
+      out[i] = (base ? base[i] : 0.0) + s;
+    }
+  }
+  __syncthreads();
+}
+
+// M (packed lower, LDS, FP32) = A diag(D) A^T on MFMA; diag0 = its diagonal.
+__device__ void assemble(const Lp& L, float* M, double reg) {
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int lrow = lane & 15, lk = lane >> 4;
+  const int T = (L.m + 15) / 16;
+  const int tiles = T * (T + 1) / 2;
+  for (int t = wv; t < tiles; t += kWaves) {
+    int ti = static_cast<int>((sqrt(8.0 * t + 1.0) - 1.0) * 0.5);
+    while ((ti + 1) * (ti + 2) / 2 <= t) ++ti;
+    while (ti * (ti + 1) / 2 > t) --ti;
+    const int tj = t - ti * (ti + 1) / 2;
+    const int r0 = ti * 16, c0 = tj * 16;
+    const int ra = r0 + lrow, cb = c0 + lrow;
+    flt4_t acc = {0.f, 0.f, 0.f, 0.f};
+    for (int k0 = 0; k0 < L.n; k0 += 16) {
+      float av[4], bv[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int k = k0 + 4 * u + lk;
+        const bool kin = k < L.n;
+        const float dk = kin ? static_cast<float>(L.D[k]) : 0.f;
+        av[u] = (kin && ra < L.m) ? L.Af[ra + static_cast<long long>(k) * L.m] * dk : 0.f;
+        bv[u] = (kin && cb < L.m) ? L.Af[cb + static_cast<long long>(k) * L.m] : 0.f;
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) acc = __builtin_amdgcn_mfma_f32_16x16x4f32(av[u], bv[u], acc, 0, 0, 0);
+    }
+    // lane holds column c0 + lrow, rows r0 + 4*lk + g
+#pragma unroll
+    for (int g = 0; g < 4; ++g) {
+      const int i = r0 + 4 * lk + g, j = c0 + lrow;
+      if (i < L.m && j < L.m && i >= j) M[packed_index(i, j, L.m)] = acc[g] + (i == j ? static_cast<float>(reg) : 0.f);
+    }
+  }
+  __syncthreads();
+  for (int i = tid; i < L.m; i += kThreads) L.diag0[i] = static_cast<double>(M[packed_index(i, i, L.m)]);
+  __syncthreads();
+}
+
+// In-place FP32 Cholesky of the packed lower triangle in LDS. Returns the
+// number of dropped pivots.
+__device__ int cholesky(const Lp& L, float* M) {
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int m = L.m;
+  int dropped = 0;
+  for (int k = 0; k < m; ++k) {
+    const long long kk = packed_index(k, k, m);
+    const double dk = static_cast<double>(M[kk]);
+    const bool drop = !(dk > 1e-13 * fabs(L.diag0[k])) && isfinite(dk);
+    const float piv = drop ? 1e18f : (isfinite(dk) ? static_cast<float>(sqrt(dk)) : 1.f);
+    dropped += drop ? 1 : 0;
+    for (int i = k + 1 + tid; i < m; i += kThreads) M[kk + (i - k)] /= piv;
+    __syncthreads();
+    if (tid == 0) M[kk] = piv;
+    for (int j = k + 1 + wv; j < m; j += kWaves) {
+      const float ljk = M[kk + (j - k)];
+      const long long cj = packed_index(j, j, m);
+      for (int i = j + lane; i < m; i += 64) M[cj + (i - j)] -= M[kk + (i - k)] * ljk;
+    }
+    __syncthreads();
+  }
+  return dropped;
+}
+
+// x <- L^-T L^-1 x in LDS (vec, FP64), with the FP32 factor.
+__device__ void precondition(const Lp& L, const float* M, double* vec, double* tmp) {
+  const int tid = threadIdx.x;
+  const int m = L.m;
+  __syncthreads();
+  for (int k = 0; k < m; ++k) {  // forward: tmp = L^-1 vec
+    const long long kk = packed_index(k, k, m);
+    const double yk = vec[k] / static_cast<double>(M[kk]);
+    if (tid == 0) tmp[k] = yk;
+    for (int i = k + 1 + tid; i < m; i += kThreads) vec[i] -= static_cast<double>(M[kk + (i - k)]) * yk;
+    __syncthreads();
+  }
+  for (int k = m - 1; k >= 0; --k) {  // backward: vec = L^-T tmp
+    const long long kk = packed_index(k, k, m);
+    const double xk = tmp[k] / static_cast<double>(M[kk]);
+    if (tid == 0) vec[k] = xk;
+    for (int j = tid; j < k; j += kThreads) tmp[j] -= static_cast<double>(M[packed_index(k, j, m)]) * xk;
+    __syncthreads();
+  }
+}
+
+// q = A D A^T p (FP64, matrix-free); p, q global (m).
+__device__ void normal_matvec(const Lp& L, const double* p, double* q, double* stage) {
+  gemv_t(L, p, L.t, stage);
+  for (int j = threadIdx.x; j < L.n; j += kThreads) L.t[j] *= L.D[j];
+  __syncthreads();
+  gemv_n(L, L.t, nullptr, q, stage);
+}
+
+// dy = M^-1 rhs by PCG. Returns iterations.
+__device__ int solve_normal(const Lp& L, const float* M, double* stage, double* red, double tol, int max_it) {
+  const int tid = threadIdx.x, m = L.m;
+  double* pv = stage;       // [0, m)  preconditioner vector (LDS)
+  double* tv = stage + 512;  // [512, 512+m) preconditioner temp (LDS)
+  // x0 = P rhs
+  for (int i = tid; i < m; i += kThreads) pv[i] = L.rhs[i];
+  precondition(L, M, pv, tv);
+  for (int i = tid; i < m; i += kThreads) L.dy[i] = pv[i];
+  __syncthreads();
+  normal_matvec(L, L.dy, L.q, stage);
+  double rn = 0.0, bn = 0.0;
+  for (int i = tid; i < m; i += kThreads) {
+    const double r = L.rhs[i] - L.q[i];
+    L.r[i] = r;
+    rn = fmax(rn, fabs(r));
+    bn = fmax(bn, fabs(L.rhs[i]));
+  }
+  rn = bred<1>(rn, red);
+  bn = bred<1>(bn, red);
+  if (!(rn > tol * fmax(bn, 1e-300))) return 0;
+  for (int i = tid; i < m; i += kThreads) pv[i] = L.r[i];
+  precondition(L, M, pv, tv);
+  double rz = 0.0;
+  for (int i = tid; i < m; i += kThreads) {
+    L.zz[i] = pv[i];
+    L.p[i] = pv[i];
+    rz += L.r[i] * pv[i];
+  }
+  rz = bred<0>(rz, red);
+  int it = 0;
+  for (it = 1; it <= max_it; ++it) {
+    __syncthreads();
+    normal_matvec(L, L.p, L.q, stage);
+    double pq = 0.0;
+    for (int i = tid; i < m; i += kThreads) pq += L.p[i] * L.q[i];
+    pq = bred<0>(pq, red);
+    const double alpha = pq != 0.0 ? rz / pq : 0.0;
+    rn = 0.0;
+    for (int i = tid; i < m; i += kThreads) {
+      L.dy[i] += alpha * L.p[i];
+      const double r = L.r[i] - alpha * L.q[i];
+      L.r[i] = r;
+      rn = fmax(rn, fabs(r));
+    }
+    rn = bred<1>(rn, red);
+    if (!(rn > tol * bn) || !isfinite(rn)) break;
+    for (int i = tid; i < m; i += kThreads) pv[i] = L.r[i];
+    precondition(L, M, pv, tv);
+    double rz2 = 0.0;
+    for (int i = tid; i < m; i += kThreads) rz2 += L.r[i] * pv[i];
+    rz2 = bred<0>(rz2, red);
+    const double beta = rz != 0.0 ? rz2 / rz : 0.0;
+    rz = rz2;
+    for (int i = tid; i < m; i += kThreads) L.p[i] = pv[i] + beta * L.p[i];
+  }
+  __syncthreads();
+  return it;
+}
+
+__device__ __forceinline__ bool active(const Lp& L, int j) { return L.u[j] > 0.0; }
+__device__ __forceinline__ bool bounded(const Lp& L, int j) { return L.u[j] < 1e29; }
+
+// Newton direction for complementarity right-hand sides r1 (x z) and r2 (w v),
+// given the current residuals; writes (dx, dw, dz, dv) and dy.
+__device__ int newton(const Lp& L, const float* M, double* stage, double* red, const double* r1, const double* r2,
+                      double* dx, double* dw, double* dz, double* dv, const BatchOpts& o) {
+  const int tid = threadIdx.x;
+  for (int j = tid; j < L.n; j += kThreads) {
+    double tj = 0.0;
+    if (active(L, j)) {
+      double g = L.rc[j] - r1[j] / L.x[j];
+      if (bounded(L, j)) g += (r2[j] - L.v[j] * L.ru[j]) / L.w[j];
+      tj = L.D[j] * g;
+    }
+    L.t[j] = tj;
+  }
+  __syncthreads();
+  gemv_n(L, L.t, L.rb, L.rhs, stage);
+  // keep t: gemv_t / normal_matvec overwrite L.t, so stash it in dx first
+  for (int j = tid; j < L.n; j += kThreads) dx[j] = L.t[j];
+  __syncthreads();
+  const int its = solve_normal(L, M, stage, red, o.pcg_tol, o.pcg_max);
+  gemv_t(L, L.dy, L.t, stage);  // t = A^T dy
+  for (int j = tid; j < L.n; j += kThreads) {
+    if (!active(L, j)) {
+      dx[j] = dw[j] = dz[j] = dv[j] = 0.0;
+      continue;
+    }
+    const double dxj = L.D[j] * L.t[j] - dx[j];
+    dx[j] = dxj;
+    dz[j] = (r1[j] - L.z[j] * dxj) / L.x[j];
+    if (bounded(L, j)) {
+      const double dwj = L.ru[j] - dxj;
+      dw[j] = dwj;
+      dv[j] = (r2[j] - L.v[j] * dwj) / L.w[j];
+    } else {
+      dw[j] = dv[j] = 0.0;
+    }
+  }
+  __syncthreads();
+  return its;
+}
+
+// Largest step in [0, 1] keeping (x, w) and (z, v) nonnegative.
+__device__ void max_steps(const Lp& L, const double* dx, const double* dw, const double* dz, const double* dv,
+                          double* red, double* ap, double* ad) {
+  double a = 1.0, b = 1.0;
+  for (int j = threadIdx.x; j < L.n; j += kThreads) {
+    if (!active(L, j)) continue;
+    if (dx[j] < 0.0) a = fmin(a, -L.x[j] / dx[j]);
+    if (dz[j] < 0.0) b = fmin(b, -L.z[j] / dz[j]);
+    if (bounded(L, j)) {
+      if (dw[j] < 0.0) a = fmin(a, -L.w[j] / dw[j]);
+      if (dv[j] < 0.0) b = fmin(b, -L.v[j] / dv[j]);
+    }
+  }
+  *ap = bred<2>(a, red);
+  *ad = bred<2>(b, red);
+}
+
+__global__ __launch_bounds__(kThreads) void batch_ipm_kernel(const BatchMatrix* __restrict__ mats,
+                                                             const BatchDesc* __restrict__ descs, int count,
+                                                             unsigned* queue, const double* __restrict__ b_all,
+                                                             const double* __restrict__ c_all,
+                                                             const double* __restrict__ u_all, double* ws,
+                                                             double* x_all, sov_batch_result_dev* out, BatchOpts o) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  double* stage = reinterpret_cast<double*>(smem);           // kStage doubles
+  double* red = stage + kStage;                              // 64 doubles
+  float* M = reinterpret_cast<float*>(red + 64);             // packed lower triangle
+  __shared__ int s_lp;
+  const int tid = threadIdx.x;
+  for (;;) {
+    __syncthreads();
+    if (tid == 0) s_lp = static_cast<int>(atomicAdd(queue, 1u));
+    __syncthreads();
+    const int lp = s_lp;
+    if (lp >= count) break;
+    const BatchDesc dsc = descs[lp];
+    const BatchMatrix mat = mats[dsc.mat];
+    Lp L;
+    L.m = mat.m;
+    L.n = mat.n;
+    L.A = mat.a64;
+    L.Af = mat.a32;
+    L.b = b_all + dsc.b_off;
+    L.c = c_all + dsc.x_off;
+    L.u = u_all + dsc.x_off;
+    double* wsp = ws + dsc.ws_off;
+    double** nv[] = {&L.x, &L.w, &L.z, &L.v, &L.rc, &L.ru, &L.D, &L.dx, &L.dw, &L.dz, &L.dv,
+                     &L.dxa, &L.dwa, &L.dza, &L.dva, &L.t};
+    for (double** pp : nv) {
+      *pp = wsp;
+      wsp += L.n;
+    }
+    double** mv[] = {&L.y, &L.rb, &L.dy, &L.r, &L.p, &L.q, &L.zz, &L.rhs, &L.diag0};
+    for (double** pp : mv) {
+      *pp = wsp;
+      wsp += L.m;
+    }
+    const int m = L.m, n = L.n;
+    // Starting point: x mid-box (or 1), w = u - x, z = v = 1, y = 0.
+    double nb = 0.0;
+    for (int j = tid; j < n; j += kThreads) {
+      const double uj = L.u[j];
+      double xj = 0.0, wj = 0.0, vj = 0.0, zj = 0.0;
+      if (uj > 0.0) {
+        xj = uj < 1e29 ? 0.5 * uj : 1.0;
+        wj = uj < 1e29 ? uj - xj : 0.0;
+        zj = 1.0;
+        vj = uj < 1e29 ? 1.0 : 0.0;
+        nb += uj < 1e29 ? 1.0 : 0.0;
+      }
+      L.x[j] = xj;
+      L.w[j] = wj;
+      L.z[j] = zj;
+      L.v[j] = vj;
+    }
+    for (int i = tid; i < m; i += kThreads) L.y[i] = 0.0;
+    double nact = 0.0;
+    for (int j = tid; j < n; j += kThreads) nact += active(L, j) ? 1.0 : 0.0;
+    nact = bred<0>(nact, red);
+    nb = bred<0>(nb, red);
+    double bnorm = 0.0, cnorm = 0.0;
+    for (int i = tid; i < m; i += kThreads) bnorm = fmax(bnorm, fabs(L.b[i]));
+    for (int j = tid; j < n; j += kThreads) cnorm = fmax(cnorm, fabs(L.c[j]));
+    bnorm = bred<1>(bnorm, red);
+    cnorm = bred<1>(cnorm, red);
+
+    int status = 5, iters = 0, pcg_total = 0;
+    double pobj = 0.0, gap = 0.0, pres = 0.0, dres = 0.0;
+    for (int it = 0; it < o.max_iter; ++it) {
+      __syncthreads();
+      // residuals
+      gemv_n(L, L.x, nullptr, L.rb, stage);  // rb = A x (then b - A x)
+      gemv_t(L, L.y, L.t, stage);            // t = A^T y
+      double rbmax = 0.0, rcmax = 0.0, rumax = 0.0, cx = 0.0, by = 0.0, uv = 0.0, xz = 0.0;
+      for (int i = tid; i < m; i += kThreads) {
+        const double r = L.b[i] - L.rb[i];
+        L.rb[i] = r;
+        rbmax = fmax(rbmax, fabs(r));
+        by += L.b[i] * L.y[i];
+      }
+      for (int j = tid; j < n; j += kThreads) {
+        if (!active(L, j)) {
+          L.rc[j] = 0.0;
+          L.ru[j] = 0.0;
+          continue;
+        }
+        const double rcj = L.c[j] - L.t[j] - L.z[j] + L.v[j];
+        L.rc[j] = rcj;
+        rcmax = fmax(rcmax, fabs(rcj));
+        cx += L.c[j] * L.x[j];
+        xz += L.x[j] * L.z[j];
+        if (bounded(L, j)) {
+          const double ruj = L.u[j] - L.x[j] - L.w[j];
+          L.ru[j] = ruj;
+          rumax = fmax(rumax, fabs(ruj));
+          uv += L.u[j] * L.v[j];
+          xz += L.w[j] * L.v[j];
+        } else {
+          L.ru[j] = 0.0;
+        }
+      }
+      rbmax = bred<1>(rbmax, red);
+      rcmax = bred<1>(rcmax, red);
+      rumax = bred<1>(rumax, red);
+      cx = bred<0>(cx, red);
+      by = bred<0>(by, red);
+      uv = bred<0>(uv, red);
+      xz = bred<0>(xz, red);
+      pobj = cx;
+      const double dobj = by - uv;
+      pres = fmax(rbmax / (1.0 + bnorm), rumax);
+      dres = rcmax / (1.0 + cnorm);
+      gap = fabs(pobj - dobj) / (1.0 + fabs(pobj));
+      iters = it;
+      if (pres < o.tol && dres < o.tol && gap < o.tol) {
+        status = 0;
+        break;
+      }
+      if (!isfinite(pobj) || !isfinite(dobj)) {
+        status = 6;
+        break;
+      }
+      const double mu = xz / fmax(nact + nb, 1.0);
+      // scaling D = (z/x + v/w)^-1
+      for (int j = tid; j < n; j += kThreads) {
+        double dj = 0.0;
+        if (active(L, j)) {
+          double g = L.z[j] / L.x[j];
+          if (bounded(L, j)) g += L.v[j] / L.w[j];
+          dj = 1.0 / g;
+        }
+        L.D[j] = dj;
+      }
+      __syncthreads();
+      assemble(L, M, 0.0);
+      cholesky(L, M);
+      // predictor: r1 = -x z, r2 = -w v (in dxa/dwa scratch: use dza/dva as rhs storage)
+      for (int j = tid; j < n; j += kThreads) {
+        L.dza[j] = -L.x[j] * L.z[j];
+        L.dva[j] = -L.w[j] * L.v[j];
+      }
+      __syncthreads();
+      // r1 -> p-buffer copies: newton reads r1/r2 after writing dz/dv, so pass copies.
+      for (int j = tid; j < n; j += kThreads) {
+        L.dxa[j] = L.dza[j];
+        L.dwa[j] = L.dva[j];
+      }
+      __syncthreads();
+      pcg_total += newton(L, M, stage, red, L.dxa, L.dwa, L.dx, L.dw, L.dz, L.dv, o);
+      double ap, ad;
+      max_steps(L, L.dx, L.dw, L.dz, L.dv, red, &ap, &ad);
+      double mua = 0.0;
+      for (int j = tid; j < n; j += kThreads) {
+        if (!active(L, j)) continue;
+        mua += (L.x[j] + ap * L.dx[j]) * (L.z[j] + ad * L.dz[j]);
+        if (bounded(L, j)) mua += (L.w[j] + ap * L.dw[j]) * (L.v[j] + ad * L.dv[j]);
+      }
+      mua = bred<0>(mua, red) / fmax(nact + nb, 1.0);
+      const double sigma = mu > 0.0 ? fmin(1.0, pow(mua / mu, 3.0)) : 0.0;
+      // corrector: r1 = sigma mu - x z - dx_a dz_a, r2 = sigma mu - w v - dw_a dv_a
+      for (int j = tid; j < n; j += kThreads) {
+        L.dxa[j] = sigma * mu - L.x[j] * L.z[j] - L.dx[j] * L.dz[j];
+        L.dwa[j] = bounded(L, j) ? sigma * mu - L.w[j] * L.v[j] - L.dw[j] * L.dv[j] : 0.0;
+      }
+      __syncthreads();
+      pcg_total += newton(L, M, stage, red, L.dxa, L.dwa, L.dx, L.dw, L.dz, L.dv, o);
+      max_steps(L, L.dx, L.dw, L.dz, L.dv, red, &ap, &ad);
+      ap = fmin(1.0, o.tau * ap);
+      ad = fmin(1.0, o.tau * ad);
+      for (int j = tid; j < n; j += kThreads) {
+        if (!active(L, j)) continue;
+        L.x[j] = fmax(L.x[j] + ap * L.dx[j], 1e-14);
+        L.z[j] = fmax(L.z[j] + ad * L.dz[j], 1e-14);
+        if (bounded(L, j)) {
+          L.w[j] = fmax(L.w[j] + ap * L.dw[j], 1e-14);
+          L.v[j] = fmax(L.v[j] + ad * L.dv[j], 1e-14);
+        }
+      }
+      for (int i = tid; i < m; i += kThreads) L.y[i] += ad * L.dy[i];
+      iters = it + 1;
+    }
+    __syncthreads();
+    if (x_all)
+      for (int j = tid; j < n; j += kThreads) x_all[dsc.x_off + j] = L.x[j];
+    if (tid == 0) {
+      sov_batch_result_dev r;
+      r.status = status;
+      r.iterations = iters;
+      r.pcg_iterations = pcg_total;
+      r.pad = 0;
+      r.objective = pobj;
+      r.duality_gap = gap;
+      r.primal_residual = pres;
+      r.dual_residual = dres;
+      out[lp] = r;
+    }
+  }
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Host side: matrices are uploaded once (FP64 and an FP32 copy for MFMA);
+// each solve ships the per-LP vectors and runs one launch.
+// ---------------------------------------------------------------------------
+struct sov_batch {
+  std::vector<BatchMatrix> mats;      // device pointers
+  std::vector<void*> owned;           // allocations to free
+  float last_ms = 0.f;
+  ~sov_batch() {
+    for (void* p : owned) (void)hipFree(p);
+  }
+};
+
+namespace {
+
+std::size_t lds_bytes(int mmax) {
+  return sizeof(double) * (kStage + 64) + sizeof(float) * static_cast<std::size_t>(mmax) * (mmax + 1) / 2;
+}
+
+template <typename T>
+T* dev_alloc(sov_batch* h, std::size_t count) {
+  void* p = nullptr;
+  if (hipMalloc(&p, std::max<std::size_t>(count, 1) * sizeof(T)) != hipSuccess) return nullptr;
+  h->owned.push_back(p);
+  return static_cast<T*>(p);
+}
+
+// Device buffers owned by one call.
+struct TmpBuffers {
+  std::vector<void*> p;
+  ~TmpBuffers() {
+    for (void* q : p) (void)hipFree(q);
+  }
+  template <typename T>
+  T* get(std::size_t cnt) {
+    void* q = nullptr;
+    if (hipMalloc(&q, std::max<std::size_t>(cnt, 1) * sizeof(T)) != hipSuccess) return nullptr;
+    p.push_back(q);
+    return static_cast<T*>(q);
+  }
+};
+
+}  // namespace
+
+extern "C" {
+
+int sov_batch_create(sov_batch** out) {
+  if (!out) return SOV_ERR_ARG;
+  *out = nullptr;
+  if (!sov::device_ready()) return SOV_ERR_NODEV;
+  static const bool lds_ok =
+      hipFuncSetAttribute(reinterpret_cast<const void*>(batch_ipm_kernel), hipFuncAttributeMaxDynamicSharedMemorySize,
+                          static_cast<int>(lds_bytes(kMaxRows))) == hipSuccess;
+  if (!lds_ok) {
+    sov::set_error("sov_batch_create: could not raise the LDS limit");
+    return SOV_ERR_DEVICE;
+  }
+  *out = new sov_batch;
+  return SOV_OK;
+}
+
+int sov_batch_add_matrix(sov_batch* h, int32_t m, int32_t n, const double* a_colmajor, int32_t* id) {
+  if (!h || !a_colmajor || !id || m <= 0 || n <= 0) return SOV_ERR_ARG;
+  if (m > kMaxRows) {
+    sov::set_error("sov_batch_add_matrix: at most 272 rows per batched LP");
+    return SOV_ERR_ARG;
+  }
+  const std::size_t cnt = static_cast<std::size_t>(m) * n;
+  std::vector<float> af(cnt);
+  for (std::size_t k = 0; k < cnt; ++k) af[k] = static_cast<float>(a_colmajor[k]);
+  double* a64 = dev_alloc<double>(h, cnt);
+  float* a32 = dev_alloc<float>(h, cnt);
+  if (!a64 || !a32) return SOV_ERR_NOMEM;
+  SOV_HIP_TRY(hipMemcpy(a64, a_colmajor, cnt * sizeof(double), hipMemcpyHostToDevice));
+  SOV_HIP_TRY(hipMemcpy(a32, af.data(), cnt * sizeof(float), hipMemcpyHostToDevice));
+  h->mats.push_back({m, n, a64, a32});
+  *id = static_cast<int32_t>(h->mats.size() - 1);
+  return SOV_OK;
+}
+
+int sov_batch_solve(sov_batch* h, int32_t count, const int32_t* matrix_id, const double* b_all,
+                    const int64_t* b_off, const double* c_all, const double* u_all, const int64_t* x_off,
+                    const sov_ipm_options* opt, sov_batch_result* res, double* x_all) {
+  if (!h || count < 0 || !matrix_id || !b_all || !b_off || !c_all || !u_all || !x_off || !opt || !res)
+    return SOV_ERR_ARG;
+  if (count == 0) return SOV_OK;
+  hipStream_t s = sov::stream();
+  std::vector<BatchDesc> descs(static_cast<std::size_t>(count));
+  long long b_total = 0, x_total = 0, ws_total = 0;
+  int mmax = 0;
+  for (int k = 0; k < count; ++k) {
+    const int id = matrix_id[k];
+    if (id < 0 || id >= static_cast<int>(h->mats.size())) return SOV_ERR_ARG;
+    const BatchMatrix& mt = h->mats[id];
+    mmax = std::max(mmax, mt.m);
+    descs[k] = {id, 0, b_off[k], x_off[k], ws_total};
+    b_total = std::max<long long>(b_total, b_off[k] + mt.m);
+    x_total = std::max<long long>(x_total, x_off[k] + mt.n);
+    ws_total += 16LL * mt.n + 9LL * mt.m;
+  }
+  TmpBuffers tmp;  // per-solve device buffers, freed at the end of the call
+  auto* d_mats = tmp.get<BatchMatrix>(h->mats.size());
+  auto* d_desc = tmp.get<BatchDesc>(descs.size());
+  auto* d_b = tmp.get<double>(static_cast<std::size_t>(b_total));
+  auto* d_c = tmp.get<double>(static_cast<std::size_t>(x_total));
+  auto* d_u = tmp.get<double>(static_cast<std::size_t>(x_total));
+  auto* d_ws = tmp.get<double>(static_cast<std::size_t>(ws_total));
+  auto* d_x = x_all ? tmp.get<double>(static_cast<std::size_t>(x_total)) : nullptr;
+  auto* d_out = tmp.get<sov_batch_result_dev>(static_cast<std::size_t>(count));
+  auto* d_queue = tmp.get<unsigned>(1);
+  if (!d_mats || !d_desc || !d_b || !d_c || !d_u || !d_ws || (x_all && !d_x) || !d_out || !d_queue)
+    return SOV_ERR_NOMEM;
+  SOV_HIP_TRY(hipMemcpyAsync(d_mats, h->mats.data(), h->mats.size() * sizeof(BatchMatrix), hipMemcpyHostToDevice, s));
+  SOV_HIP_TRY(hipMemcpyAsync(d_desc, descs.data(), descs.size() * sizeof(BatchDesc), hipMemcpyHostToDevice, s));
+  SOV_HIP_TRY(hipMemcpyAsync(d_b, b_all, b_total * sizeof(double), hipMemcpyHostToDevice, s));
+  SOV_HIP_TRY(hipMemcpyAsync(d_c, c_all, x_total * sizeof(double), hipMemcpyHostToDevice, s));
+  SOV_HIP_TRY(hipMemcpyAsync(d_u, u_all, x_total * sizeof(double), hipMemcpyHostToDevice, s));
+  SOV_HIP_TRY(hipMemsetAsync(d_queue, 0, sizeof(unsigned), s));
+  BatchOpts o;
+  o.tol = std::min(opt->feasibility_tol, opt->optimality_tol);
+  o.tau = opt->fraction_to_boundary;
+  o.max_iter = opt->max_iterations > 0 ? std::min(opt->max_iterations, 200) : 100;
+  o.pcg_tol = opt->refine_tol > 0 ? opt->refine_tol : 1e-12;
+  o.pcg_max = opt->refine_max > 0 ? opt->refine_max : 30;
+  int dev = 0, cus = 0;
+  SOV_HIP_TRY(hipGetDevice(&dev));
+  SOV_HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+  const unsigned grid = static_cast<unsigned>(std::min(count, std::max(cus, 1)));
+  hipEvent_t e0, e1;
+  SOV_HIP_TRY(hipEventCreate(&e0));
+  SOV_HIP_TRY(hipEventCreate(&e1));
+  (void)hipEventRecord(e0, s);
+  batch_ipm_kernel<<<grid, kThreads, lds_bytes(mmax), s>>>(d_mats, d_desc, count, d_queue, d_b, d_c, d_u, d_ws, d_x,
+                                                          d_out, o);
+  const hipError_t le = hipGetLastError();
+  (void)hipEventRecord(e1, s);
+  if (le != hipSuccess) {
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    sov::set_error(std::string("batch_ipm_kernel launch: ") + hipGetErrorString(le));
+    return SOV_ERR_DEVICE;
+  }
+  std::vector<sov_batch_result_dev> host(static_cast<std::size_t>(count));
+  SOV_HIP_TRY(hipMemcpyAsync(host.data(), d_out, count * sizeof(sov_batch_result_dev), hipMemcpyDeviceToHost, s));
+  if (x_all) SOV_HIP_TRY(hipMemcpyAsync(x_all, d_x, x_total * sizeof(double), hipMemcpyDeviceToHost, s));
+  SOV_HIP_TRY(hipStreamSynchronize(s));
+  (void)hipEventElapsedTime(&h->last_ms, e0, e1);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  for (int k = 0; k < count; ++k) {
+    res[k].status = host[k].status;
+    res[k].iterations = host[k].iterations;
+    res[k].pcg_iterations = host[k].pcg_iterations;
+    res[k].objective = host[k].objective;
+    res[k].duality_gap = host[k].duality_gap;
+    res[k].primal_residual = host[k].primal_residual;
+    res[k].dual_residual = host[k].dual_residual;
+  }
+  sov::count_operation();
+  return SOV_OK;
+}
+
+double sov_batch_last_kernel_ms(const sov_batch* h) { return h ? h->last_ms : -1.0; }
+
+int sov_batch_free(sov_batch* h) {
+  delete h;
+  return SOV_OK;
+}
+
+}  // extern "C"

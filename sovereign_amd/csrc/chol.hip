@@ -1,4 +1,4 @@
-// Blocked right-looking Cholesky on MFMA and persistent triangular solves.
+// Blocked right-looking Cholesky on MFMA and blocked triangular solves.
 //
 // Replaces, for the interior-point normal equations, the reference's dense
 // path (build_normal_eq + DenseLU, solver/lp/src/interior_point.cpp:74-101,
@@ -10,16 +10,17 @@
 //
 // Per panel of kPanel columns:
 //   potrf_panel  one workgroup factors the diagonal block in LDS;
-//   trsm_panel   L21 = A21 L11^-T, 64 rows per workgroup, L11 in LDS;
+//   gemm (store) L21 = A21 inv(L11)^T on MFMA, copied back into the factor;
 //   gemm_update  A22 -= L21 L21^T on MFMA (lower tiles only).
-// After the last panel the 64x64 diagonal blocks are inverted in parallel
-// so the triangular solves become products.
+// The panel kernel also inverts its diagonal block; the inverses serve the
+// panel TRSM (as a GEMM) and the blocked triangular solves.
 
 #include "chol.hpp"
 #include "mfma_gemm.hpp"
 
 #include <algorithm>
 #include <atomic>
+#include <vector>
 
 namespace sov {
 namespace {
@@ -29,145 +30,154 @@ std::atomic<bool> g_timing{false};
 constexpr int kPanelLd = kPanel + 1;  // LDS leading dimension of a panel block
 
 // ---------------------------------------------------------------------------
-// Diagonal block: right-looking unblocked Cholesky in LDS, 1024 threads.
+// Diagonal block: Cholesky of the nb x nb block (nb <= kPanel = 128) and the
+// inverse of its factor, register-tiled. The 1024 threads form a 32 x 32 grid
+// of 4 x 4 tiles held in registers; per column only the finished column of L
+// goes through LDS (Lst, which ends up holding all of L), so LDS traffic is a
+// broadcast column per step rather than the whole trailing block. The inverse
+// X = L^-1 is the forward substitution L X = I, right-looking on the same
+// tiles. L11 goes back to `a`, X to inv_out (leading dimension kPanel, zero
+// above the diagonal), which turns the panel TRSM into a GEMM.
 // ---------------------------------------------------------------------------
 template <typename T>
 __global__ __launch_bounds__(1024) void potrf_panel_kernel(T* __restrict__ a, long long lda, int k0, int nb,
                                                            const double* __restrict__ diag0,
-                                                           FactorFlags* __restrict__ flags) {
+                                                           FactorFlags* __restrict__ flags, T* __restrict__ inv_out) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  T* L = reinterpret_cast<T*>(smem_raw);
-  double* d0 = reinterpret_cast<double*>(smem_raw + sizeof(T) * kPanel * kPanelLd);
+  T* Lst = reinterpret_cast<T*>(smem_raw);           // [kPanel][kPanelLd], column j at j*kPanelLd
+  T* rowbuf = Lst + kPanel * kPanelLd;               // [2][kPanel]
+  double* d0 = reinterpret_cast<double*>(rowbuf + 2 * kPanel);
+  double* dv = d0 + kPanel;  // running diagonal (see below)
   const int tid = threadIdx.x;
-  for (int idx = tid; idx < nb * nb; idx += blockDim.x) {
-    const int i = idx % nb, j = idx / nb;
-    if (i >= j) L[j * kPanelLd + i] = a[(k0 + i) + static_cast<long long>(k0 + j) * lda];
-  }
+  const int tr = tid & 31, tc = tid >> 5;  // tile row / tile column
+  const int r0 = 4 * tr, c0 = 4 * tc;
+  const bool lower_tile = tr >= tc;
+  for (int idx = tid; idx < kPanel * kPanelLd; idx += blockDim.x) Lst[idx] = T(0);
   if (tid < nb) d0[tid] = diag0[k0 + tid];
+  T v[4][4];
+#pragma unroll
+  for (int r = 0; r < 4; ++r)
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      const int i = r0 + r, k = c0 + c;
+      v[r][c] = (lower_tile && i >= k && i < nb && k < nb) ? a[(k0 + i) + static_cast<long long>(k0 + k) * lda]
+                                                          : T(0);
+    }
+  __syncthreads();
+  // dv[i] tracks the diagonal entry i as the elimination proceeds (column
+  // owners update it while publishing their column), so every thread can
+  // take the pivot decision itself and a column step needs one barrier.
+  for (int i = tid; i < kPanel; i += blockDim.x)
+    dv[i] = i < nb ? static_cast<double>(a[(k0 + i) + static_cast<long long>(k0 + i) * lda]) : 1.0;
   __syncthreads();
   for (int j = 0; j < nb; ++j) {
-    // Every thread takes the same pivot decision from the same LDS value.
-    const double dj = static_cast<double>(L[j * kPanelLd + j]);
+    const int jt = j >> 2, jr = j & 3;
+    const double dj = dv[j];
     const bool finite = isfinite(dj);
     const bool drop = finite && dj <= 1e-13 * fabs(d0[j]);
     const bool bad = !drop && (!finite || dj < 1e-30);
     const T piv = drop ? DropPivot<T>::value : bad ? T(1) : static_cast<T>(sqrt(dj));
-    if (tid > j && tid < nb) L[j * kPanelLd + tid] /= piv;
-    __syncthreads();
+    const T rpiv = T(1) / piv;
     if (tid == 0) {
-      L[j * kPanelLd + j] = piv;
       if (drop) atomicAdd(&flags->replaced, 1);
       if (bad) atomicOr(&flags->status, 1);
     }
-    const int w = nb - j - 1;
-    for (int idx = tid; idx < w * w; idx += blockDim.x) {
-      const int i = j + 1 + idx % w, t = j + 1 + idx / w;
-      if (i >= t) L[t * kPanelLd + i] -= L[j * kPanelLd + i] * L[j * kPanelLd + t];
-    }
-    __syncthreads();
-  }
-  for (int idx = tid; idx < nb * nb; idx += blockDim.x) {
-    const int i = idx % nb, j = idx / nb;
-    if (i >= j) a[(k0 + i) + static_cast<long long>(k0 + j) * lda] = L[j * kPanelLd + i];
-  }
-}
-
-// ---------------------------------------------------------------------------
-// Panel rows below the diagonal block: L21 = A21 L11^-T by forward
-// substitution along the row. 1024 threads = 16 wavefronts; lane = row
-// (64 rows per workgroup), wavefront w owns columns w, w+16, ...; the value
-// of column j is passed through LDS (double-buffered, one barrier per column).
-// ---------------------------------------------------------------------------
-template <typename T>
-__global__ __launch_bounds__(1024) void trsm_panel_kernel(T* __restrict__ a, long long lda, int k0, int nb,
-                                                          int m) {
-  extern __shared__ __align__(16) unsigned char smem_raw[];
-  T* L = reinterpret_cast<T*>(smem_raw);
-  T* X = L + kPanel * kPanelLd;  // [2][64]
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int row = k0 + nb + blockIdx.x * 64 + lane;
-  const bool live = row < m;
-  for (int idx = tid; idx < nb * nb; idx += blockDim.x) {
-    const int i = idx % nb, j = idx / nb;
-    if (i >= j) L[j * kPanelLd + i] = a[(k0 + i) + static_cast<long long>(k0 + j) * lda];
-  }
-  constexpr int Q = kPanel / 16;
-  T v[Q];
+    if (tc == jt && lower_tile) {  // column owners scale and publish column j
 #pragma unroll
-  for (int q = 0; q < Q; ++q) {
-    const int c = wave + 16 * q;
-    v[q] = (live && c < nb) ? a[row + static_cast<long long>(k0 + c) * lda] : T(0);
-  }
-  __syncthreads();
-  for (int j = 0; j < nb; ++j) {
-    const int owner = j & 15, qj = j >> 4;
-    T* xb = X + (j & 1) * 64;
-    if (wave == owner) {
+      for (int c = 0; c < 4; ++c) {
+        if (c != jr) continue;
 #pragma unroll
-      for (int q = 0; q < Q; ++q) {
-        if (q == qj) {
-          v[q] = v[q] / L[j * kPanelLd + j];
-          xb[lane] = v[q];
+        for (int r = 0; r < 4; ++r) {
+          const int i = r0 + r;
+          if (i == j) {
+            v[r][c] = piv;
+            Lst[j * kPanelLd + j] = piv;
+          } else if (i > j && i < nb) {
+            const T l = v[r][c] * rpiv;
+            v[r][c] = l;
+            Lst[j * kPanelLd + i] = l;
+            dv[i] -= static_cast<double>(l) * static_cast<double>(l);
+          }
         }
       }
     }
     __syncthreads();
-    const T xj = xb[lane];
+    if (lower_tile && c0 + 3 > j) {  // rank-1 update of the trailing triangle
+      T li[4], lk[4];
 #pragma unroll
-    for (int q = 0; q < Q; ++q) {
-      const int c = wave + 16 * q;
-      if (c > j && c < nb) v[q] -= xj * L[j * kPanelLd + c];
+      for (int r = 0; r < 4; ++r) li[r] = Lst[j * kPanelLd + r0 + r];
+#pragma unroll
+      for (int c = 0; c < 4; ++c) lk[c] = Lst[j * kPanelLd + c0 + c];
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+#pragma unroll
+        for (int c = 0; c < 4; ++c)
+          if (c0 + c > j && r0 + r >= c0 + c) v[r][c] -= li[r] * lk[c];
     }
   }
-  if (live) {
+  // L11 back to global memory.
+  if (lower_tile) {
 #pragma unroll
-    for (int q = 0; q < Q; ++q) {
-      const int c = wave + 16 * q;
-      if (c < nb) a[row + static_cast<long long>(k0 + c) * lda] = v[q];
+    for (int c = 0; c < 4; ++c)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int i = r0 + r, k = c0 + c;
+        if (i >= k && i < nb && k < nb) a[(k0 + i) + static_cast<long long>(k0 + k) * lda] = v[r][c];
+      }
+  }
+  // X = L^-1 by right-looking substitution on the identity (X stays lower).
+#pragma unroll
+  for (int r = 0; r < 4; ++r)
+#pragma unroll
+    for (int c = 0; c < 4; ++c) v[r][c] = (r0 + r == c0 + c) ? T(1) : T(0);
+  for (int k = 0; k < nb; ++k) {
+    T* rb = rowbuf + (k & 1) * kPanel;
+    const T lkk = Lst[k * kPanelLd + k];
+    if (tr == (k >> 2)) {  // owners of row k scale it and publish it
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        if (r != (k & 3)) continue;
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+          v[r][c] = v[r][c] / lkk;
+          rb[c0 + c] = v[r][c];
+        }
+      }
+    }
+    __syncthreads();
+    if (lower_tile && r0 + 3 > k) {
+      T xr[4];
+#pragma unroll
+      for (int c = 0; c < 4; ++c) xr[c] = rb[c0 + c];
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int i = r0 + r;
+        if (i <= k) continue;
+        const T lik = Lst[k * kPanelLd + i];
+#pragma unroll
+        for (int c = 0; c < 4; ++c) v[r][c] -= lik * xr[c];
+      }
     }
   }
+#pragma unroll
+  for (int c = 0; c < 4; ++c)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int i = r0 + r, k = c0 + c;
+      inv_out[i + k * kPanel] = (lower_tile && i >= k && i < nb && k < nb) ? v[r][c] : T(0);
+    }
 }
 
-// ---------------------------------------------------------------------------
-// Inverse of each kSolveBlock diagonal block of L (in-place lower-triangular
-// inversion in LDS, LAPACK trti2 order), written to dinv (block p at
-// p*64*64, column-major, zero above the diagonal).
-// ---------------------------------------------------------------------------
+
+// L21 = A21 inv(L11)^T, computed out of place into P (rows below the
+// diagonal block, leading dimension lda), copied back into the factor here.
 template <typename T>
-__global__ __launch_bounds__(256) void diag_inverse_kernel(const T* __restrict__ a, long long lda, int m,
-                                                           T* __restrict__ dinv) {
-  constexpr int B = kSolveBlock, LD = B + 1;
-  __shared__ double W[B * LD];
-  const int p = blockIdx.x, base = p * B;
-  const int nbp = min(B, m - base);
-  const int tid = threadIdx.x;
-  for (int idx = tid; idx < B * B; idx += blockDim.x) {
-    const int i = idx % B, j = idx / B;
-    double v = 0.0;
-    if (i < nbp && j < nbp && i >= j) v = static_cast<double>(a[(base + i) + static_cast<long long>(base + j) * lda]);
-    if (i == j && i >= nbp) v = 1.0;
-    W[j * LD + i] = v;
-  }
-  __syncthreads();
-  for (int j = B - 1; j >= 0; --j) {
-    const double inv = 1.0 / W[j * LD + j];
-    // new column below the diagonal: -inv * Winv[j+1:, j+1:] * W[j+1:, j]
-    double out = 0.0;
-    const int i = tid;
-    if (i > j && i < B) {
-      for (int k = j + 1; k <= i; ++k) out += W[k * LD + i] * W[j * LD + k];
-      out *= -inv;
-    }
-    __syncthreads();
-    if (i > j && i < B) W[j * LD + i] = out;
-    if (tid == 0) W[j * LD + j] = inv;
-    __syncthreads();
-  }
-  T* dst = dinv + static_cast<long long>(p) * B * B;
-  for (int idx = tid; idx < B * B; idx += blockDim.x) {
-    const int i = idx % B, j = idx / B;
-    dst[idx] = i >= j ? static_cast<T>(W[j * LD + i]) : T(0);
-  }
+__global__ __launch_bounds__(256) void copy_panel_kernel(const T* __restrict__ P, T* __restrict__ a, long long lda,
+                                                         int r0, int k0, int rows, int nb) {
+  const long long idx = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (idx >= static_cast<long long>(rows) * nb) return;
+  const int r = static_cast<int>(idx % rows), c = static_cast<int>(idx / rows);
+  a[(r0 + r) + static_cast<long long>(k0 + c) * lda] = P[r + static_cast<long long>(c) * lda];
 }
 
 __global__ void reset_flags_kernel(FactorFlags* f) {
@@ -175,161 +185,11 @@ __global__ void reset_flags_kernel(FactorFlags* f) {
   f->replaced = 0;
 }
 
-// Bounded wait for a row block published in generation `gen`. A wait that
-// exceeds the budget marks the solve failed (status 2) instead of hanging.
-__device__ __forceinline__ void wait_block(const unsigned* flag, unsigned gen, FactorFlags* fl) {
-  long long spins = 0;
-  while (__hip_atomic_load(flag, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) != gen) {
-    __builtin_amdgcn_s_sleep(2);
-    if (++spins > (1LL << 26)) {
-      atomicOr(&fl->status, 2);
-      return;
-    }
-  }
-}
-
-__device__ __forceinline__ double load_fresh(const double* p) {
-  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-// ---------------------------------------------------------------------------
-// Forward solve L y = b in place on x, persistent: one workgroup per row
-// block, block order taken from a ticket so a workgroup only ever waits on
-// blocks whose workgroups started earlier. 256 threads: lane = row of the
-// block, wavefront h = a quarter of the 64 columns of each source block.
-// ---------------------------------------------------------------------------
-template <typename T>
-__global__ __launch_bounds__(256) void trsv_forward_kernel(const T* __restrict__ a, long long lda, int m,
-                                                           const T* __restrict__ dinv, double* x,
-                                                           unsigned* blk_flags, FactorFlags* fl, unsigned gen,
-                                                           unsigned ticket_base) {
-  constexpr int B = kSolveBlock;
-  __shared__ int s_blk;
-  __shared__ double yj[B];
-  __shared__ double part[4][B];
-  const int tid = threadIdx.x, lane = tid & 63, h = tid >> 6;
-  if (tid == 0) s_blk = static_cast<int>(atomicAdd(&fl->ticket, 1u) - ticket_base);
-  __syncthreads();
-  const int i = s_blk;
-  const int row = i * B + lane;
-  const bool live = row < m;
-  double acc = 0.0;
-  for (int j = 0; j < i; ++j) {
-    if (tid == 0) wait_block(&blk_flags[j], gen, fl);
-    __syncthreads();
-    if (tid < B) yj[tid] = load_fresh(&x[j * B + tid]);
-    __syncthreads();
-    if (live) {
-#pragma unroll 4
-      for (int q = 0; q < 16; ++q) {
-        const int c = h * 16 + q;
-        acc += static_cast<double>(a[row + static_cast<long long>(j * B + c) * lda]) * yj[c];
-      }
-    }
-    __syncthreads();
-  }
-  part[h][lane] = acc;
-  __syncthreads();
-  if (tid < B) {
-    const double bval = live ? x[row] : 0.0;
-    yj[tid] = bval - (part[0][tid] + part[1][tid] + part[2][tid] + part[3][tid]);
-  }
-  __syncthreads();
-  // y_i = inv(L_ii) c: row `lane`, columns of quarter h.
-  const T* inv = dinv + static_cast<long long>(i) * B * B;
-  double y = 0.0;
-#pragma unroll 4
-  for (int q = 0; q < 16; ++q) {
-    const int c = h * 16 + q;
-    y += static_cast<double>(inv[lane + c * B]) * yj[c];
-  }
-  part[h][lane] = y;
-  __syncthreads();
-  if (tid < B && live) x[row] = part[0][tid] + part[1][tid] + part[2][tid] + part[3][tid];
-  __threadfence();
-  __syncthreads();
-  if (tid == 0) __hip_atomic_store(&blk_flags[i], gen, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-// ---------------------------------------------------------------------------
-// Backward solve L^T x = y in place, persistent, blocks from the last one
-// up. Block i needs column block i of L below the diagonal: wavefront w owns
-// 16 of its columns and its lanes run down the rows, so every load is a
-// contiguous 512-byte column segment; lane partial sums are reduced once at
-// the end.
-// ---------------------------------------------------------------------------
-template <typename T>
-__global__ __launch_bounds__(256) void trsv_backward_kernel(const T* __restrict__ a, long long lda, int m,
-                                                            const T* __restrict__ dinv, double* x,
-                                                            unsigned* blk_flags, FactorFlags* fl, unsigned gen,
-                                                            unsigned ticket_base) {
-  constexpr int B = kSolveBlock;
-  __shared__ int s_blk;
-  __shared__ double xj[B];
-  __shared__ double c[B];
-  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-  const int nblk = (m + B - 1) / B;
-  if (tid == 0) s_blk = nblk - 1 - static_cast<int>(atomicAdd(&fl->ticket, 1u) - ticket_base);
-  __syncthreads();
-  const int i = s_blk;
-  double p[16];
-#pragma unroll
-  for (int q = 0; q < 16; ++q) p[q] = 0.0;
-  for (int j = nblk - 1; j > i; --j) {
-    if (tid == 0) wait_block(&blk_flags[j], gen, fl);
-    __syncthreads();
-    const int r = j * B + lane;
-    if (tid < B) xj[tid] = (j * B + tid < m) ? load_fresh(&x[j * B + tid]) : 0.0;
-    __syncthreads();
-    if (r < m) {
-      const double xr = xj[lane];
-#pragma unroll
-      for (int q = 0; q < 16; ++q) {
-        const int col = i * B + w * 16 + q;
-        p[q] += static_cast<double>(a[r + static_cast<long long>(col) * lda]) * xr;
-      }
-    }
-    __syncthreads();
-  }
-#pragma unroll
-  for (int q = 0; q < 16; ++q) {
-    const double s = wave_sum(p[q]);
-    if (lane == 0) c[w * 16 + q] = s;
-  }
-  __syncthreads();
-  if (tid < B) {
-    const int r = i * B + tid;
-    c[tid] = (r < m ? x[r] : 0.0) - c[tid];
-  }
-  __syncthreads();
-  // x_i = inv(L_ii)^T c: output k = w*16+q sums inv[r + k*B] c[r] over r = lane.
-  const T* inv = dinv + static_cast<long long>(i) * B * B;
-  double outv[16];
-#pragma unroll
-  for (int q = 0; q < 16; ++q) {
-    const int k = w * 16 + q;
-    outv[q] = wave_sum(static_cast<double>(inv[lane + k * B]) * c[lane]);
-  }
-  if (lane == 0) {
-#pragma unroll
-    for (int q = 0; q < 16; ++q) {
-      const int r = i * B + w * 16 + q;
-      if (r < m) x[r] = outv[q];
-    }
-  }
-  __threadfence();
-  __syncthreads();
-  if (tid == 0) __hip_atomic_store(&blk_flags[i], gen, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-}
+#include "trsv.hpp"
 
 template <typename T>
 constexpr std::size_t panel_lds_bytes() {
-  return sizeof(T) * kPanel * kPanelLd + sizeof(double) * kPanel;
-}
-
-template <typename T>
-constexpr std::size_t trsm_lds_bytes() {
-  return sizeof(T) * kPanel * kPanelLd + sizeof(T) * 128;
+  return sizeof(T) * (kPanel * kPanelLd + 2 * kPanel) + sizeof(double) * 2 * kPanel;
 }
 
 template <typename T>
@@ -338,31 +198,55 @@ bool configure_lds() {
     bool r = hipFuncSetAttribute(reinterpret_cast<const void*>(potrf_panel_kernel<T>),
                                  hipFuncAttributeMaxDynamicSharedMemorySize,
                                  static_cast<int>(panel_lds_bytes<T>())) == hipSuccess;
-    r = r && hipFuncSetAttribute(reinterpret_cast<const void*>(trsm_panel_kernel<T>),
-                                 hipFuncAttributeMaxDynamicSharedMemorySize,
-                                 static_cast<int>(trsm_lds_bytes<T>())) == hipSuccess;
     return r;
   }();
   return ok;
 }
 
-struct EventPair {
+// Event pairs recorded around kernels while timing is on. Recording does
+// not wait; the elapsed times are collected (after one stream sync) by
+// sov_kernel_times, so timing can stay on inside a timed region.
+enum TimedKind { kTimeSyrk = 0, kTimePanel = 1, kTimeSolve = 2 };
+struct TimedPair {
   hipEvent_t a = nullptr, b = nullptr;
-  EventPair() {
-    (void)hipEventCreate(&a);
-    (void)hipEventCreate(&b);
-  }
-  ~EventPair() {
-    if (a) (void)hipEventDestroy(a);
-    if (b) (void)hipEventDestroy(b);
-  }
-  float ms() {
-    float t = 0.f;
-    (void)hipEventSynchronize(b);
-    (void)hipEventElapsedTime(&t, a, b);
-    return t;
+  int kind = 0;
+  double flops = 0.0;
+  int launches = 0;
+};
+struct EventPool {
+  std::vector<TimedPair> pairs;
+  std::size_t used = 0;
+  TimedPair* next() {
+    if (used == pairs.size()) {
+      TimedPair p;
+      if (hipEventCreate(&p.a) != hipSuccess || hipEventCreate(&p.b) != hipSuccess) return nullptr;
+      pairs.push_back(p);
+    }
+    return &pairs[used++];
   }
 };
+EventPool& pool() {
+  static EventPool p;
+  return p;
+}
+
+// Opens a timed pair on `s` when timing is on (nullptr otherwise).
+TimedPair* timed_begin(hipStream_t s, int kind) {
+  if (!g_timing.load()) return nullptr;
+  TimedPair* p = pool().next();
+  if (!p) return nullptr;
+  p->kind = kind;
+  p->flops = 0.0;
+  p->launches = 0;
+  (void)hipEventRecord(p->a, s);
+  return p;
+}
+void timed_end(TimedPair* p, hipStream_t s, int launches, double flops) {
+  if (!p) return;
+  p->launches = launches;
+  p->flops = flops;
+  (void)hipEventRecord(p->b, s);
+}
 
 }  // namespace
 
@@ -373,30 +257,56 @@ KernelTimes& kernel_times() {
   return t;
 }
 
+// Folds the recorded pairs into the totals (waits for the stream once).
+void collect_kernel_times(hipStream_t s) {
+  EventPool& p = pool();
+  if (p.used == 0) return;
+  (void)hipStreamSynchronize(s);
+  KernelTimes& t = kernel_times();
+  for (std::size_t k = 0; k < p.used; ++k) {
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, p.pairs[k].a, p.pairs[k].b) != hipSuccess) ms = 0.f;
+    const TimedPair& e = p.pairs[k];
+    if (e.kind == kTimeSyrk) {
+      t.syrk_ms += ms;
+      t.syrk_launches += e.launches;
+      t.syrk_flops += e.flops;
+    } else if (e.kind == kTimePanel) {
+      t.panel_ms += ms;
+      t.panel_launches += e.launches;
+    } else {
+      t.solve_ms += ms;
+    }
+  }
+  p.used = 0;
+}
+
 template <typename T>
 DenseCholesky<T>::~DenseCholesky() {
-  if (dinv_) (void)hipFree(dinv_);
-  if (solve_flags_) (void)hipFree(solve_flags_);
-  if (flags_) (void)hipFree(flags_);
+  if (side_) (void)hipStreamSynchronize(side_);
+  if (ev_panel_) (void)hipEventDestroy(ev_panel_);
+  if (ev_bulk_) (void)hipEventDestroy(ev_bulk_);
+  if (side_) (void)hipStreamDestroy(side_);
+  for (void* p : {static_cast<void*>(panel_), static_cast<void*>(panel_inv_), static_cast<void*>(work_),
+                  static_cast<void*>(flags_)})
+    if (p) (void)hipFree(p);
 }
 
 template <typename T>
 int DenseCholesky<T>::reserve(int m) {
   if (m <= m_ && flags_) return SOV_OK;
-  if (dinv_) (void)hipFree(dinv_);
-  if (solve_flags_) (void)hipFree(solve_flags_);
-  if (flags_) (void)hipFree(flags_);
-  dinv_ = nullptr;
-  solve_flags_ = nullptr;
+  for (void* p : {static_cast<void*>(panel_), static_cast<void*>(panel_inv_), static_cast<void*>(work_),
+                  static_cast<void*>(flags_)})
+    if (p) (void)hipFree(p);
+  panel_ = panel_inv_ = nullptr;
+  work_ = nullptr;
   flags_ = nullptr;
-  const int nblk = (m + kSolveBlock - 1) / kSolveBlock;
-  SOV_HIP_TRY(hipMalloc(&dinv_, sizeof(T) * static_cast<std::size_t>(nblk) * kSolveBlock * kSolveBlock));
-  SOV_HIP_TRY(hipMalloc(&solve_flags_, sizeof(unsigned) * static_cast<std::size_t>(nblk)));
+  const int np = (m + kPanel - 1) / kPanel;
+  SOV_HIP_TRY(hipMalloc(&panel_, sizeof(T) * static_cast<std::size_t>(m) * kPanel * 2));
+  SOV_HIP_TRY(hipMalloc(&panel_inv_, sizeof(T) * static_cast<std::size_t>(np) * kPanel * kPanel));
+  SOV_HIP_TRY(hipMalloc(&work_, sizeof(double) * static_cast<std::size_t>(m)));
   SOV_HIP_TRY(hipMalloc(&flags_, sizeof(FactorFlags)));
-  SOV_HIP_TRY(hipMemset(solve_flags_, 0, sizeof(unsigned) * static_cast<std::size_t>(nblk)));
   SOV_HIP_TRY(hipMemset(flags_, 0, sizeof(FactorFlags)));
-  generation_ = 0;
-  tickets_used_ = 0;
   m_ = m;
   if (!configure_lds<T>()) {
     set_error("could not raise the LDS limit of the panel kernels");
@@ -409,64 +319,81 @@ template <typename T>
 int DenseCholesky<T>::factor(T* a, int m, const double* diag0, hipStream_t s) {
   int rc = reserve(m);
   if (rc != SOV_OK) return rc;
-  const bool timing = kernel_timing();
-  EventPair ev;
+  if (!side_) {
+    SOV_HIP_TRY(hipStreamCreateWithFlags(&side_, hipStreamNonBlocking));
+    SOV_HIP_TRY(hipEventCreateWithFlags(&ev_panel_, hipEventDisableTiming));
+    SOV_HIP_TRY(hipEventCreateWithFlags(&ev_bulk_, hipEventDisableTiming));
+  }
   reset_flags_kernel<<<1, 1, 0, s>>>(flags_);
   SOV_LAUNCH_CHECK();
+  // The side stream starts after everything already queued on s.
+  SOV_HIP_TRY(hipEventRecord(ev_bulk_, s));
+  SOV_HIP_TRY(hipStreamWaitEvent(side_, ev_bulk_, 0));
   const long long lda = m;
-  for (int k0 = 0; k0 < m; k0 += kPanel) {
+  const std::size_t pstride = static_cast<std::size_t>(m) * kPanel;
+  for (int k0 = 0, p = 0; k0 < m; k0 += kPanel, ++p) {
     const int nb = std::min(kPanel, m - k0);
-    if (timing) (void)hipEventRecord(ev.a, s);
-    potrf_panel_kernel<T><<<1, 1024, panel_lds_bytes<T>(), s>>>(a, lda, k0, nb, diag0, flags_);
-    SOV_LAUNCH_CHECK();
     const int below = m - k0 - nb;
+    T* pinv = panel_inv_ + static_cast<long long>(p) * kPanel * kPanel;
+    T* L21 = panel_ + (p & 1) * pstride;
+    TimedPair* tp = timed_begin(s, kTimePanel);
+    potrf_panel_kernel<T><<<1, 1024, panel_lds_bytes<T>(), s>>>(a, lda, k0, nb, diag0, flags_, pinv);
+    SOV_LAUNCH_CHECK();
     if (below > 0) {
-      trsm_panel_kernel<T><<<blocks_for(below, 64), 1024, trsm_lds_bytes<T>(), s>>>(a, lda, k0, nb, m);
+      // L21 = A21 inv(L11)^T on MFMA, out of place, then back into the factor.
+      const T* a21 = a + (k0 + nb) + static_cast<long long>(k0) * lda;
+      SOV_HIP_TRY((gemm_update<T, false, false, true>(below, nb, nb, a21, lda, pinv, kPanel, L21, lda, T(1), s)));
+      copy_panel_kernel<T><<<blocks_for(static_cast<long long>(below) * nb, 256), 256, 0, s>>>(L21, a, lda, k0 + nb,
+                                                                                               k0, below, nb);
       SOV_LAUNCH_CHECK();
     }
-    if (timing) {
-      (void)hipEventRecord(ev.b, s);
-      kernel_times().panel_ms += ev.ms();
-      kernel_times().panel_launches += below > 0 ? 2 : 1;
-    }
-    if (below > 0) {
-      const T* l21 = a + (k0 + nb) + static_cast<long long>(k0) * lda;
-      T* a22 = a + (k0 + nb) + static_cast<long long>(k0 + nb) * lda;
-      if (timing) (void)hipEventRecord(ev.a, s);
-      SOV_HIP_TRY((gemm_update<T, false, true>(below, below, nb, l21, lda, l21, lda, a22, lda, T(-1), s)));
-      if (timing) {
-        (void)hipEventRecord(ev.b, s);
-        KernelTimes& kt = kernel_times();
-        kt.syrk_ms += ev.ms();
-        kt.syrk_launches += 1;
-        kt.syrk_flops += static_cast<double>(below) * (below + 1) * nb;
-      }
+    timed_end(tp, s, below > 0 ? 3 : 1, 0.0);
+    if (below <= 0) continue;
+    // Look-ahead: the next panel's column strip first (on s, the critical
+    // path), the rest of the trailing matrix on the side stream, where it
+    // overlaps the next panel's factorization. The strip update also writes
+    // the strictly upper part of its diagonal block, which nothing reads.
+    const int strip = std::min(kPanel, below);
+    const int rest = below - strip;
+    SOV_HIP_TRY(hipStreamWaitEvent(s, ev_bulk_, 0));  // the strip was part of the previous bulk update
+    TimedPair* ts = timed_begin(s, kTimeSyrk);
+    T* a_strip = a + (k0 + nb) + static_cast<long long>(k0 + nb) * lda;
+    SOV_HIP_TRY((gemm_update<T, false, false>(below, strip, nb, L21, lda, L21, lda, a_strip, lda, T(-1), s)));
+    timed_end(ts, s, 1, static_cast<double>(below) * strip * nb * 2.0 - static_cast<double>(strip) * (strip - 1) * nb);
+    if (rest > 0) {
+      SOV_HIP_TRY(hipEventRecord(ev_panel_, s));
+      SOV_HIP_TRY(hipStreamWaitEvent(side_, ev_panel_, 0));
+      T* a22 = a + (k0 + nb + strip) + static_cast<long long>(k0 + nb + strip) * lda;
+      const T* l21r = L21 + strip;
+      TimedPair* tb = timed_begin(side_, kTimeSyrk);
+      SOV_HIP_TRY((gemm_update<T, false, true>(rest, rest, nb, l21r, lda, l21r, lda, a22, lda, T(-1), side_)));
+      timed_end(tb, side_, 1, static_cast<double>(rest) * (rest + 1) * nb);
+      SOV_HIP_TRY(hipEventRecord(ev_bulk_, side_));
     }
   }
-  diag_inverse_kernel<T><<<blocks_for(m, kSolveBlock), 256, 0, s>>>(a, lda, m, dinv_);
-  SOV_LAUNCH_CHECK();
+  SOV_HIP_TRY(hipEventRecord(ev_bulk_, side_));
+  SOV_HIP_TRY(hipStreamWaitEvent(s, ev_bulk_, 0));
   return SOV_OK;
 }
 
 template <typename T>
 int DenseCholesky<T>::solve(const T* a, int m, double* x, double* /*work*/, hipStream_t s) {
   if (m > m_) return SOV_ERR_ARG;
-  const unsigned nblk = static_cast<unsigned>((m + kSolveBlock - 1) / kSolveBlock);
-  const bool timing = kernel_timing();
-  EventPair ev;
-  if (timing) (void)hipEventRecord(ev.a, s);
-  unsigned gen = ++generation_;
-  trsv_forward_kernel<T><<<nblk, 256, 0, s>>>(a, m, m, dinv_, x, solve_flags_, flags_, gen, tickets_used_);
-  SOV_LAUNCH_CHECK();
-  tickets_used_ += nblk;
-  gen = ++generation_;
-  trsv_backward_kernel<T><<<nblk, 256, 0, s>>>(a, m, m, dinv_, x, solve_flags_, flags_, gen, tickets_used_);
-  SOV_LAUNCH_CHECK();
-  tickets_used_ += nblk;
-  if (timing) {
-    (void)hipEventRecord(ev.b, s);
-    kernel_times().solve_ms += ev.ms();
+  const int np = (m + kPanel - 1) / kPanel;
+  TimedPair* tp = timed_begin(s, kTimeSolve);
+  for (int p = 0; p < np; ++p) {  // L y = x: y into work_, x updated in place
+    const int below = std::max(0, m - (p + 1) * kPanel);
+    trsv_fwd_step_kernel<T><<<std::max(1u, blocks_for(below, kSolveRows)), 256, 0, s>>>(a, m, m, p, panel_inv_, x,
+                                                                                         work_);
   }
+  SOV_LAUNCH_CHECK();
+  for (int p = np - 1; p >= 0; --p) {  // L^T x = y: x back into x, work_ updated in place
+    const int above = p * kPanel;
+    trsv_bwd_step_kernel<T><<<std::max(1u, blocks_for(above, kSolveRows)), 256, 0, s>>>(a, m, m, p, panel_inv_, work_,
+                                                                                         x);
+  }
+  SOV_LAUNCH_CHECK();
+  timed_end(tp, s, 2 * np, 0.0);
   return SOV_OK;
 }
 
@@ -481,6 +408,7 @@ void sov_set_kernel_timing(int32_t on) { sov::set_kernel_timing(on != 0); }
 
 int sov_kernel_times(double* out) {
   if (!out) return SOV_ERR_ARG;
+  if (sov::device_ready()) sov::collect_kernel_times(sov::stream());
   sov::KernelTimes& t = sov::kernel_times();
   const double v[7] = {t.syrk_ms, t.syrk_launches, t.syrk_flops, t.panel_ms, t.solve_ms, t.assemble_ms,
                        t.panel_launches};

@@ -18,15 +18,12 @@ struct DropPivot<float> {
   static constexpr float value = 1e18f;
 };
 
-constexpr int kPanel = 128;  // factorization panel width (K of the trailing update)
-constexpr int kSolveBlock = 64;  // row block of the triangular solves
+constexpr int kPanel = 128;  // factorization panel width (K of the trailing update, solve block)
 
 // Per-factorization device state shared by the kernels.
 struct FactorFlags {
   int status;    // 0 ok, 1 breakdown (non-finite or < 1e-30 pivot)
   int replaced;  // dropped pivots
-  unsigned ticket;  // persistent triangular-solve work counter (monotone)
-  unsigned pad;
 };
 
 // Owns the factor workspace for an m x m matrix stored column-major in `a`
@@ -35,7 +32,7 @@ template <typename T>
 class DenseCholesky {
  public:
   ~DenseCholesky();
-  int reserve(int m);  // allocates dinv, flags, solve flags
+  int reserve(int m);  // allocates the panel inverses, scratch and flags
   // In-place factorization of a (lower); diag0 holds the diagonal before
   // elimination (for the drop test). Asynchronous; status in flags().
   int factor(T* a, int m, const double* diag0, hipStream_t s);
@@ -45,11 +42,12 @@ class DenseCholesky {
 
  private:
   int m_ = 0;
-  T* dinv_ = nullptr;         // inverses of the kSolveBlock diagonal blocks
-  unsigned* solve_flags_ = nullptr;  // per row block: generation it was published in
+  T* panel_ = nullptr;      // L21 of two consecutive panels (2 x m x kPanel, leading dimension m)
+  hipStream_t side_ = nullptr;          // trailing updates run here, overlapping the next panel
+  hipEvent_t ev_panel_ = nullptr, ev_bulk_ = nullptr;
+  T* panel_inv_ = nullptr;  // inverses of all diagonal blocks (kPanel x kPanel each)
+  double* work_ = nullptr;  // intermediate vector of the solves (m)
   FactorFlags* flags_ = nullptr;
-  unsigned generation_ = 0;   // solve launches so far (each publishes all blocks once)
-  unsigned tickets_used_ = 0;
 };
 
 // Global switch for per-kernel event timing (adds synchronisation) and the
@@ -61,5 +59,6 @@ struct KernelTimes {
 void set_kernel_timing(bool on);
 bool kernel_timing();
 KernelTimes& kernel_times();
+void collect_kernel_times(hipStream_t s);
 
 }  // namespace sov

@@ -18,6 +18,8 @@
 #include <cmath>
 #include <limits>
 #include <vector>
+#include <cstdio>
+#include <cstdlib>
 
 namespace sov {
 namespace {
@@ -370,6 +372,8 @@ int ipm_run(sov_ipm* h, const double* b_h, const double* c_h, const sov_ipm_opti
   const double solve_tol = opt.refine_tol > 0 ? opt.refine_tol : 1e-12;
   const int solve_max = opt.refine_max > 0 ? opt.refine_max : 30;
   Events ev;
+  const char* trace_env = std::getenv("SOVEREIGN_IPM_TRACE");
+  const bool trace = trace_env && trace_env[0] == '1';
   double factor_ms = 0.0, solve_ms = 0.0;
 
   auto launch_rows_axpy = [&](const double* v, const double* base, double* out) {
@@ -481,6 +485,10 @@ int ipm_run(sov_ipm* h, const double* b_h, const double* c_h, const sov_ipm_opti
       }
     }
     const IpmScalars S = *sc_h;
+    if (trace)
+      std::fprintf(stderr, "[ipm] it=%d prec=%s pres=%.3e dres=%.3e gap=%.3e mu=%.3e ap=%.3f ad=%.3f pcg=%d res=%.2e\n",
+                   it, cur_prec == SOV_PREC_FP32 ? "fp32" : "fp64", S.pres, S.dres, S.gap, S.mu, S.ap, S.ad,
+                   res->refine_iterations, res->max_solve_residual);
     const double metric = std::max(S.pres, std::max(S.dres, S.gap));
     if (metric < best_metric) {
       best_metric = metric;

@@ -35,6 +35,9 @@ struct Mfma<double> {
   __device__ static __forceinline__ acc_t step(double a, double b, acc_t c) {
     return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
   }
+  // Output row held by accumulator register g of a lane in lane group lk
+  // (lane >> 4): D(i, j) sits on lane j + 16 (i mod 4), register i / 4.
+  __device__ static __forceinline__ int row(int lk, int g) { return 4 * g + lk; }
 };
 
 template <>
@@ -43,6 +46,8 @@ struct Mfma<float> {
   __device__ static __forceinline__ acc_t step(float a, float b, acc_t c) {
     return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
   }
+  // D(i, j) sits on lane j + 16 (i / 4), register i mod 4.
+  __device__ static __forceinline__ int row(int lk, int g) { return 4 * lk + g; }
 };
 
 constexpr int kTile = 64;
@@ -70,7 +75,7 @@ __device__ __forceinline__ void tile_coords(int t, bool lower, int tiles_m, int*
   }
 }
 
-template <typename T, bool NN, bool LOWER>
+template <typename T, bool NN, bool LOWER, bool STORE = false>
 __global__ __launch_bounds__(256) void gemm_update_kernel(int M, int N, int K, const T* __restrict__ A,
                                                           long long lda, const T* __restrict__ B,
                                                           long long ldb, T* __restrict__ C,
@@ -143,7 +148,9 @@ __global__ __launch_bounds__(256) void gemm_update_kernel(int M, int N, int K, c
     }
   }
 
-  // Lane holds column (lane & 15) and rows 4*(lane >> 4) + g of each block.
+  // Lane holds column (lane & 15) of each block; the rows of its registers
+  // depend on the instruction (Mfma<T>::row). Operands: A(i, k) and B(k, j)
+  // sit on lane i + 16 k and j + 16 k.
 #pragma unroll
   for (int i = 0; i < 2; ++i) {
 #pragma unroll
@@ -152,22 +159,24 @@ __global__ __launch_bounds__(256) void gemm_update_kernel(int M, int N, int K, c
       if (gc >= N) continue;
 #pragma unroll
       for (int g = 0; g < 4; ++g) {
-        const int gr = r0 + wr + 16 * i + 4 * lk + g;
+        const int gr = r0 + wr + 16 * i + Mfma<T>::row(lk, g);
         if (gr >= M || (LOWER && gr < gc)) continue;
-        C[gr + static_cast<long long>(gc) * ldc] += alpha * acc[i][j][g];
+        if (STORE) C[gr + static_cast<long long>(gc) * ldc] = alpha * acc[i][j][g];
+        else C[gr + static_cast<long long>(gc) * ldc] += alpha * acc[i][j][g];
       }
     }
   }
 }
 
 // Host launcher. Returns a hipError_t from the launch.
-template <typename T, bool NN, bool LOWER>
+template <typename T, bool NN, bool LOWER, bool STORE = false>
 inline hipError_t gemm_update(int M, int N, int K, const T* A, long long lda, const T* B, long long ldb, T* C,
                               long long ldc, T alpha, hipStream_t s) {
   if (M <= 0 || N <= 0 || K <= 0) return hipSuccess;
   const long long tm = (M + kTile - 1) / kTile, tn = (N + kTile - 1) / kTile;
   const long long tiles = LOWER ? tm * (tm + 1) / 2 : tm * tn;
-  gemm_update_kernel<T, NN, LOWER><<<static_cast<unsigned>(tiles), 256, 0, s>>>(M, N, K, A, lda, B, ldb, C, ldc, alpha);
+  gemm_update_kernel<T, NN, LOWER, STORE><<<static_cast<unsigned>(tiles), 256, 0, s>>>(M, N, K, A, lda, B, ldb, C, ldc,
+                                                                                     alpha);
   return hipGetLastError();
 }
 

@@ -175,6 +175,21 @@ def staircase(products: int, periods: int) -> LpModel:
                    name=f"staircase_{products}x{periods}")
 
 
+def multi_knapsack(n: int = 500, m: int = 200, seed: int = BASE_SEED) -> LpModel:
+    """0/1 multidimensional knapsack (maximize): weights uniform int [1, 100],
+    capacity floor(0.5 x row sum), profit_j = mean_i w_ij + U[0, 20]; m <= rows,
+    binaries as 0 <= x <= 1 (integrality is the branch-and-bound's business)."""
+    rng = np.random.default_rng(seed)
+    W = rng.integers(1, 101, size=(m, n)).astype(np.float64)
+    cap = np.floor(0.5 * W.sum(axis=1))
+    profit = W.mean(axis=0) + rng.uniform(0.0, 20.0, size=n)
+    ptr = np.arange(0, m * n + 1, m, dtype=np.int32)
+    rows = np.tile(np.arange(m, dtype=np.int32), n)
+    return LpModel(n_vars=n, n_rows=m, col_ptr=ptr, row_idx=rows, values=W.T.reshape(-1), obj=profit,
+                   lb=np.zeros(n), ub=np.ones(n), row_sense=b"L" * m, rhs=cap, maximize=True,
+                   name=f"mkp_{n}x{m}")
+
+
 @dataclass
 class FormLp:
     """An LP already in interior-point form: min c'x, A x = b, x >= 0."""

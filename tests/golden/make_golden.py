@@ -57,17 +57,21 @@ def json_to_model(path: str, name: str) -> P.LpModel:
         obj[index[var]] = float(val)
     lb = np.array([float(v.get("lower_bound", 0.0)) for v in d["variables"]])
     ub = np.array([float(v.get("upper_bound", 1e30)) for v in d["variables"]])
-    return P.LpModel(n_vars=nv, n_rows=nr, col_ptr=ptr, row_idx=rows, values=vals, obj=obj, lb=lb, ub=ub,
-                     row_sense="".join(senses), rhs=np.array(rhs),
-                     obj_constant=float(d["objective"].get("constant", 0.0)),
-                     maximize=str(d.get("sense", "minimize")).lower().startswith("max"), name=name)
+    model = P.LpModel(n_vars=nv, n_rows=nr, col_ptr=ptr, row_idx=rows, values=vals, obj=obj, lb=lb, ub=ub,
+                      row_sense="".join(senses), rhs=np.array(rhs),
+                      obj_constant=float(d["objective"].get("constant", 0.0)),
+                      maximize=str(d.get("sense", "minimize")).lower().startswith("max"), name=name)
+    model.is_int = np.array([v.get("type", "continuous") in ("binary", "integer") for v in d["variables"]],
+                            dtype=np.int8)
+    return model
 
 
 def save_model(model: P.LpModel, path: str):
     np.savez_compressed(path, col_ptr=model.col_ptr, row_idx=model.row_idx, values=model.values, obj=model.obj,
                         lb=model.lb, ub=model.ub, row_sense=np.frombuffer(model.row_sense, dtype=np.uint8),
                         rhs=model.rhs, scalars=np.array([model.n_vars, model.n_rows, int(model.maximize)]),
-                        obj_constant=np.array([model.obj_constant]))
+                        obj_constant=np.array([model.obj_constant]),
+                        is_int=getattr(model, "is_int", np.zeros(model.n_vars, dtype=np.int8)))
 
 
 def load_model(path: str, name: str) -> P.LpModel:
@@ -108,6 +112,16 @@ def main():
         models.append((f"fixture:{name}", m))
     for m in synthetic_models():
         models.append((f"synthetic:{m.name}", m))
+    # MILP fixtures of the reference; optimal objectives as published in its
+    # benchmarks/reports/EVIDENCE.md:47-50 (OPTIMAL 102 and 133).
+    milp = {}
+    for rel, name, published in (("miplib/multi_knapsack_18x3.json", "multi_knapsack_18x3", 102.0),
+                                 ("miplib/multi_knapsack_24x4.json", "multi_knapsack_24x4", 133.0)):
+        m = json_to_model(os.path.join(args.reference, "benchmarks", "datasets", rel), name)
+        save_model(m, os.path.join(HERE, "data", f"{name}.npz"))
+        milp[name] = {"objective": published, "source": "benchmarks/reports/EVIDENCE.md:47-50"}
+    with open(os.path.join(HERE, "milp_golden.json"), "w") as fh:
+        json.dump(milp, fh, indent=1, sort_keys=True)
     out = {}
     for key, m in models:
         for mode in ("auto", "dense", "sparse"):

@@ -14,9 +14,16 @@ GOLDEN = os.path.join(HERE, "golden")
 def load_fixture(name: str) -> P.LpModel:
     z = np.load(os.path.join(GOLDEN, "data", f"{name}.npz"), allow_pickle=False)
     nv, nr, mx = (int(v) for v in z["scalars"])
-    return P.LpModel(n_vars=nv, n_rows=nr, col_ptr=z["col_ptr"], row_idx=z["row_idx"], values=z["values"],
-                     obj=z["obj"], lb=z["lb"], ub=z["ub"], row_sense=z["row_sense"].tobytes(), rhs=z["rhs"],
-                     obj_constant=float(z["obj_constant"][0]), maximize=bool(mx), name=name)
+    model = P.LpModel(n_vars=nv, n_rows=nr, col_ptr=z["col_ptr"], row_idx=z["row_idx"], values=z["values"],
+                      obj=z["obj"], lb=z["lb"], ub=z["ub"], row_sense=z["row_sense"].tobytes(), rhs=z["rhs"],
+                      obj_constant=float(z["obj_constant"][0]), maximize=bool(mx), name=name)
+    model.is_int = z["is_int"] if "is_int" in z.files else np.zeros(nv, dtype=np.int8)
+    return model
+
+
+def milp_expected() -> dict:
+    with open(os.path.join(GOLDEN, "milp_golden.json")) as fh:
+        return json.load(fh)
 
 
 def synthetic(name: str) -> P.LpModel:
