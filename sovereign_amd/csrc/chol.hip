@@ -31,141 +31,241 @@ constexpr int kPanelLd = kPanel + 1;  // LDS leading dimension of a panel block
 
 // ---------------------------------------------------------------------------
 // Diagonal block: Cholesky of the nb x nb block (nb <= kPanel = 128) and the
-// inverse of its factor, register-tiled. The 1024 threads form a 32 x 32 grid
-// of 4 x 4 tiles held in registers; per column only the finished column of L
-// goes through LDS (Lst, which ends up holding all of L), so LDS traffic is a
-// broadcast column per step rather than the whole trailing block. The inverse
-// X = L^-1 is the forward substitution L X = I, right-looking on the same
-// tiles. L11 goes back to `a`, X to inv_out (leading dimension kPanel, zero
-// above the diagonal), which turns the panel TRSM into a GEMM.
+// inverse X = L^-1 of its factor. The block lives in LDS as S (column-major,
+// leading dimension kPanelLd): L in the lower triangle, X^T in the strict
+// upper triangle (X(i, k) at S[i * kPanelLd + k] for i > k) and the diagonal
+// of X in xdiag. It is processed in sub-panels of kSub = 32 columns:
+//   A  wavefront 0 factors the 32 x 32 diagonal block with no workgroup
+//      barrier: lanes 0-31 each hold a row of the block, lanes 32-63 each a
+//      column of its inverse, and both halves run the same right-looking
+//      update against the newly finished column of L, read from LDS;
+//   B  rows below it inside the block: L(r, sub) = A(r, sub) inv(L_ss)^T;
+//   C  the rank-32 update of the trailing lower part of the block;
+//   Y/X the inverse blocks left of the diagonal in this block row,
+//      X(t, <t) = -inv(L_tt) (L(t, <t) X(<t, <t)).
+// The pivot rule is the reference LDL^T's: a pivot <= 1e-13 x |its diagonal
+// before elimination| is dropped (L_jj = DropPivot), a non-finite or
+// < 1e-30 pivot is a breakdown. L11 goes back to `a`, X to inv_out and X^T to
+// invt_out (leading dimension kPanel, zero outside the nb x nb lower
+// triangle); they make the panel TRSM a GEMM and the triangular solves
+// matrix-vector products.
 // ---------------------------------------------------------------------------
+constexpr int kSub = 32;
+constexpr int kPanelThreads = 512;
+constexpr int kSubLd = kSub + 1;
+
 template <typename T>
-__global__ __launch_bounds__(1024) void potrf_panel_kernel(T* __restrict__ a, long long lda, int k0, int nb,
-                                                           const double* __restrict__ diag0,
-                                                           FactorFlags* __restrict__ flags, T* __restrict__ inv_out) {
+__device__ __forceinline__ T read_lane(T v, int lane);
+template <>
+__device__ __forceinline__ double read_lane<double>(double v, int lane) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane(static_cast<int>(b), lane);
+  const int hi = __builtin_amdgcn_readlane(static_cast<int>(b >> 32), lane);
+  return __longlong_as_double((static_cast<long long>(hi) << 32) | static_cast<unsigned int>(lo));
+}
+template <>
+__device__ __forceinline__ float read_lane<float>(float v, int lane) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
+}
+
+template <typename T>
+__global__ __launch_bounds__(kPanelThreads) void potrf_panel_kernel(T* __restrict__ a, long long lda, int k0, int nb,
+                                                                    const double* __restrict__ diag0,
+                                                                    FactorFlags* __restrict__ flags,
+                                                                    T* __restrict__ inv_out, T* __restrict__ invt_out) {
+  typedef typename Mfma<T>::acc_t acc_t;
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  T* Lst = reinterpret_cast<T*>(smem_raw);           // [kPanel][kPanelLd], column j at j*kPanelLd
-  T* rowbuf = Lst + kPanel * kPanelLd;               // [2][kPanel]
-  double* d0 = reinterpret_cast<double*>(rowbuf + 2 * kPanel);
-  double* dv = d0 + kPanel;  // running diagonal (see below)
-  const int tid = threadIdx.x;
-  const int tr = tid & 31, tc = tid >> 5;  // tile row / tile column
-  const int r0 = 4 * tr, c0 = 4 * tc;
-  const bool lower_tile = tr >= tc;
-  for (int idx = tid; idx < kPanel * kPanelLd; idx += blockDim.x) Lst[idx] = T(0);
-  if (tid < nb) d0[tid] = diag0[k0 + tid];
-  T v[4][4];
+  T* S = reinterpret_cast<T*>(smem_raw);                   // [kPanel][kPanelLd]
+  T* Yt = S + kPanel * kPanelLd;                           // [kPanel - kSub][kSubLd], Y(v, kk) at kk * kSubLd + v
+  T* xdiag = Yt + (kPanel - kSub) * kSubLd;                // [kPanel]
+  double* d0 = reinterpret_cast<double*>(xdiag + kPanel);  // [kPanel]
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int li = lane & 15, lk = lane >> 4;  // MFMA operand row / k lane group
+  constexpr int kWaves = kPanelThreads / 64;
+  {
+    const int i = tid & (kPanel - 1), k1 = tid >> 7;
+    T v[kPanel / 4];
 #pragma unroll
-  for (int r = 0; r < 4; ++r)
-#pragma unroll
-    for (int c = 0; c < 4; ++c) {
-      const int i = r0 + r, k = c0 + c;
-      v[r][c] = (lower_tile && i >= k && i < nb && k < nb) ? a[(k0 + i) + static_cast<long long>(k0 + k) * lda]
-                                                          : T(0);
+    for (int q = 0; q < kPanel / 4; ++q) {
+      const int k = k1 + 4 * q;
+      v[q] = (k < nb && i >= k && i < nb) ? a[(k0 + i) + static_cast<long long>(k0 + k) * lda] : T(0);
     }
+#pragma unroll
+    for (int q = 0; q < kPanel / 4; ++q) S[(k1 + 4 * q) * kPanelLd + i] = v[q];
+    if (tid < nb) d0[tid] = diag0[k0 + tid];
+  }
   __syncthreads();
-  // dv[i] tracks the diagonal entry i as the elimination proceeds (column
-  // owners update it while publishing their column), so every thread can
-  // take the pivot decision itself and a column step needs one barrier.
-  for (int i = tid; i < kPanel; i += blockDim.x)
-    dv[i] = i < nb ? static_cast<double>(a[(k0 + i) + static_cast<long long>(k0 + i) * lda]) : 1.0;
-  __syncthreads();
-  for (int j = 0; j < nb; ++j) {
-    const int jt = j >> 2, jr = j & 3;
-    const double dj = dv[j];
-    const bool finite = isfinite(dj);
-    const bool drop = finite && dj <= 1e-13 * fabs(d0[j]);
-    const bool bad = !drop && (!finite || dj < 1e-30);
-    const T piv = drop ? DropPivot<T>::value : bad ? T(1) : static_cast<T>(sqrt(dj));
-    const T rpiv = T(1) / piv;
-    if (tid == 0) {
-      if (drop) atomicAdd(&flags->replaced, 1);
-      if (bad) atomicOr(&flags->status, 1);
+
+  for (int c = 0; c < nb; c += kSub) {
+    const int sb = min(kSub, nb - c);  // width of this sub-panel
+    const int rb = nb - c - sb;        // rows below it (sb == kSub when rb > 0)
+    T* D = S + c * (kPanelLd + 1);     // its diagonal block: element (i, k) at D[k * kPanelLd + i]
+
+    // ---- A: diagonal block and its inverse in the registers of wavefront 0.
+    // Lane me < 32 holds row me of the block, lane 32 + me column me of the
+    // inverse; column j of L is broadcast from the lanes that own it.
+    if (wave == 0) {
+      const bool fac = lane < kSub;
+      const int me = lane & (kSub - 1);
+      T reg[kSub];
+#pragma unroll
+      for (int k = 0; k < kSub; ++k)
+        reg[k] = fac ? ((me < sb && k <= me) ? D[k * kPanelLd + me] : T(0)) : (k == me ? T(1) : T(0));
+      const double thr = me < sb ? 1e-13 * fabs(d0[c + me]) : 0.0;  // drop threshold of column me
+#pragma unroll
+      for (int j = 0; j < kSub; ++j) {
+        if (j < sb) {
+          const double dj = static_cast<double>(read_lane(reg[j], j));
+          const bool finite = isfinite(dj);
+          const bool drop = finite && dj <= read_lane(thr, j);
+          const bool bad = !drop && (!finite || dj < 1e-30);
+          // 1/sqrt(dj): hardware estimate and two Newton steps; sqrt = dj / sqrt(dj)
+          double rs = __builtin_amdgcn_rsq(dj);
+          rs = rs * (1.5 - 0.5 * dj * rs * rs);
+          rs = rs * (1.5 - 0.5 * dj * rs * rs);
+          const T piv = drop ? DropPivot<T>::value : bad ? T(1) : static_cast<T>(dj * rs);
+          const T rpiv = drop ? T(1) / DropPivot<T>::value : bad ? T(1) : static_cast<T>(rs);
+          if (lane == 0) {
+            if (drop) atomicAdd(&flags->replaced, 1);
+            if (bad) atomicOr(&flags->status, 1);
+            xdiag[c + j] = rpiv;
+          }
+          // factor lanes below j: l = a * rpiv; lane j: the pivot; inverse
+          // lanes: x_j *= rpiv. Then reg[k] -= reg[j] L(k, j) for k > j,
+          // L(k, j) read from lane k.
+          T v = reg[j] * rpiv;
+          if (fac && me == j) v = piv;
+          if (!fac || me >= j) reg[j] = v;
+#pragma unroll
+          for (int k = j + 1; k < kSub; ++k) reg[k] -= reg[j] * read_lane(reg[j], k);
+        }
+      }
+      if (fac) {  // row me of L
+#pragma unroll
+        for (int k = 0; k < kSub; ++k)
+          if (k <= me && me < sb) D[k * kPanelLd + me] = reg[k];
+      } else {  // X(i, me) for i > me, into the upper triangle
+#pragma unroll
+        for (int i = 1; i < kSub; ++i)
+          if (i > me && i < sb) D[i * kPanelLd + me] = reg[i];
+      }
     }
-    if (tc == jt && lower_tile) {  // column owners scale and publish column j
+    __syncthreads();
+
+    // ---- B (compute): L(r, sub) = A(r, sub) inv(L_ss)^T as 16 x 16 MFMA
+    // tiles; Y = L(t, <t) X(<t, <t) into Yt.
+    const int nrt = (rb + 15) >> 4;  // 16-row tiles below the sub-panel
+    const int nbt = 2 * nrt;         // B tiles (two 16-column tiles each row)
+    const int nyt = 2 * (c >> 4);    // Y tiles
+    acc_t bacc[3];
 #pragma unroll
-      for (int c = 0; c < 4; ++c) {
-        if (c != jr) continue;
+    for (int n = 0; n < 3; ++n) {  // tiles t = wave + n kWaves (nbt + nyt <= 24)
+      const int t = wave + n * kWaves;
+      if (t >= nbt + nyt) break;
+      acc_t acc = acc_t{0, 0, 0, 0};
+      if (t < nbt) {
+        const int r = c + sb + 16 * (t >> 1) + li;  // A row of this lane
+        const int q = 16 * (t & 1) + li;            // B column of this lane
+#pragma unroll 4
+        for (int k = 0; k < kSub; k += 4) {
+          const int kk = k + lk;
+          const T av = r < nb ? S[(c + kk) * kPanelLd + r] : T(0);
+          const T bv = kk < q ? D[q * kPanelLd + kk] : (kk == q ? xdiag[c + q] : T(0));
+          acc = Mfma<T>::step(av, bv, acc);
+        }
+        bacc[n] = acc;
+      } else {
+        const int u = t - nbt;
+        const int v = 16 * (u & 1) + li;  // A row (block row t of L)
+        const int x0 = 16 * (u >> 1);     // first output column
+        const int kx = x0 + li;           // B column of this lane
+        for (int j0 = x0; j0 < c; j0 += 16) {  // X(j, kx) = 0 for j < x0
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          const int i = r0 + r;
-          if (i == j) {
-            v[r][c] = piv;
-            Lst[j * kPanelLd + j] = piv;
-          } else if (i > j && i < nb) {
-            const T l = v[r][c] * rpiv;
-            v[r][c] = l;
-            Lst[j * kPanelLd + i] = l;
-            dv[i] -= static_cast<double>(l) * static_cast<double>(l);
+          for (int j = j0; j < j0 + 16; j += 4) {
+            const int jj = j + lk;
+            const T av = v < sb ? S[jj * kPanelLd + c + v] : T(0);
+            const T bv = jj > kx ? S[jj * kPanelLd + kx] : (jj == kx ? xdiag[kx] : T(0));
+            acc = Mfma<T>::step(av, bv, acc);
           }
         }
+#pragma unroll
+        for (int g = 0; g < 4; ++g) Yt[(x0 + li) * kSubLd + 16 * (u & 1) + Mfma<T>::row(lk, g)] = acc[g];
       }
     }
     __syncthreads();
-    if (lower_tile && c0 + 3 > j) {  // rank-1 update of the trailing triangle
-      T li[4], lk[4];
+
+    // ---- B (store); X(t, <t) = -inv(L_tt) Y into the upper triangle ----
 #pragma unroll
-      for (int r = 0; r < 4; ++r) li[r] = Lst[j * kPanelLd + r0 + r];
+    for (int n = 0; n < 3; ++n) {
+      const int t = wave + n * kWaves;
+      if (t >= nbt + nyt) break;
+      if (t < nbt) {
+        const acc_t acc = bacc[n];
+        const int q = 16 * (t & 1) + li;
 #pragma unroll
-      for (int c = 0; c < 4; ++c) lk[c] = Lst[j * kPanelLd + c0 + c];
+        for (int g = 0; g < 4; ++g) {
+          const int r = c + sb + 16 * (t >> 1) + Mfma<T>::row(lk, g);
+          if (r < nb) S[(c + q) * kPanelLd + r] = acc[g];
+        }
+      } else {
+        const int u = t - nbt;
+        const int i = 16 * (u & 1) + li;  // A row: inv(L_tt) row
+        const int x0 = 16 * (u >> 1);
+        acc_t acc = acc_t{0, 0, 0, 0};
+#pragma unroll 4
+        for (int vv = 0; vv < kSub; vv += 4) {
+          const int v = vv + lk;
+          const T av = i < sb ? (v < i ? D[i * kPanelLd + v] : (v == i ? xdiag[c + i] : T(0))) : T(0);
+          const T bv = Yt[(x0 + li) * kSubLd + v];
+          acc = Mfma<T>::step(av, bv, acc);
+        }
 #pragma unroll
-      for (int r = 0; r < 4; ++r)
-#pragma unroll
-        for (int c = 0; c < 4; ++c)
-          if (c0 + c > j && r0 + r >= c0 + c) v[r][c] -= li[r] * lk[c];
-    }
-  }
-  // L11 back to global memory.
-  if (lower_tile) {
-#pragma unroll
-    for (int c = 0; c < 4; ++c)
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int i = r0 + r, k = c0 + c;
-        if (i >= k && i < nb && k < nb) a[(k0 + i) + static_cast<long long>(k0 + k) * lda] = v[r][c];
-      }
-  }
-  // X = L^-1 by right-looking substitution on the identity (X stays lower).
-#pragma unroll
-  for (int r = 0; r < 4; ++r)
-#pragma unroll
-    for (int c = 0; c < 4; ++c) v[r][c] = (r0 + r == c0 + c) ? T(1) : T(0);
-  for (int k = 0; k < nb; ++k) {
-    T* rb = rowbuf + (k & 1) * kPanel;
-    const T lkk = Lst[k * kPanelLd + k];
-    if (tr == (k >> 2)) {  // owners of row k scale it and publish it
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        if (r != (k & 3)) continue;
-#pragma unroll
-        for (int c = 0; c < 4; ++c) {
-          v[r][c] = v[r][c] / lkk;
-          rb[c0 + c] = v[r][c];
+        for (int g = 0; g < 4; ++g) {
+          const int ii = 16 * (u & 1) + Mfma<T>::row(lk, g);
+          if (ii < sb) S[(c + ii) * kPanelLd + x0 + li] = -acc[g];
         }
       }
     }
     __syncthreads();
-    if (lower_tile && r0 + 3 > k) {
-      T xr[4];
+
+    // ---- C: rank-32 update of the trailing lower block (lower tiles) ----
+    const int nct = nrt * (nrt + 1) / 2;
+    for (int t = wave; t < nct; t += kWaves) {
+      int ti = 0;
+      while ((ti + 1) * (ti + 2) / 2 <= t) ++ti;
+      const int tj = t - ti * (ti + 1) / 2;
+      const int r = c + sb + 16 * ti + li, q = c + sb + 16 * tj + li;
+      acc_t acc = acc_t{0, 0, 0, 0};
+#pragma unroll 4
+      for (int k = 0; k < kSub; k += 4) {
+        const T* col = S + (c + k + lk) * kPanelLd;
+        const T av = r < nb ? col[r] : T(0);
+        const T bv = q < nb ? col[q] : T(0);
+        acc = Mfma<T>::step(av, bv, acc);
+      }
 #pragma unroll
-      for (int c = 0; c < 4; ++c) xr[c] = rb[c0 + c];
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int i = r0 + r;
-        if (i <= k) continue;
-        const T lik = Lst[k * kPanelLd + i];
-#pragma unroll
-        for (int c = 0; c < 4; ++c) v[r][c] -= lik * xr[c];
+      for (int g = 0; g < 4; ++g) {
+        const int rr = c + sb + 16 * ti + Mfma<T>::row(lk, g);
+        if (rr < nb && q < nb && rr >= q) S[q * kPanelLd + rr] -= acc[g];
       }
     }
+    __syncthreads();
   }
-#pragma unroll
-  for (int c = 0; c < 4; ++c)
-#pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      const int i = r0 + r, k = c0 + c;
-      inv_out[i + k * kPanel] = (lower_tile && i >= k && i < nb && k < nb) ? v[r][c] : T(0);
+
+  // L11 to the factor, X and X^T to the inverse buffers.
+  {
+    const int i = tid & (kPanel - 1), k1 = tid >> 7;
+#pragma unroll 8
+    for (int q = 0; q < kPanel / 4; ++q) {
+      const int k = k1 + 4 * q;
+      const bool in = i < nb && k < nb;
+      if (in && i >= k) a[(k0 + i) + static_cast<long long>(k0 + k) * lda] = S[k * kPanelLd + i];
+      const T x = in ? (i > k ? S[i * kPanelLd + k] : (i == k ? xdiag[i] : T(0))) : T(0);
+      const T y = in ? (k > i ? S[k * kPanelLd + i] : (k == i ? xdiag[i] : T(0))) : T(0);
+      inv_out[i + k * kPanel] = x;
+      invt_out[i + k * kPanel] = y;
     }
+  }
 }
 
 
@@ -189,7 +289,7 @@ __global__ void reset_flags_kernel(FactorFlags* f) {
 
 template <typename T>
 constexpr std::size_t panel_lds_bytes() {
-  return sizeof(T) * (kPanel * kPanelLd + 2 * kPanel) + sizeof(double) * 2 * kPanel;
+  return sizeof(T) * (kPanel * kPanelLd + (kPanel - kSub) * kSubLd + kPanel) + sizeof(double) * kPanel;
 }
 
 template <typename T>
@@ -287,23 +387,24 @@ DenseCholesky<T>::~DenseCholesky() {
   if (ev_panel_) (void)hipEventDestroy(ev_panel_);
   if (ev_bulk_) (void)hipEventDestroy(ev_bulk_);
   if (side_) (void)hipStreamDestroy(side_);
-  for (void* p : {static_cast<void*>(panel_), static_cast<void*>(panel_inv_), static_cast<void*>(work_),
-                  static_cast<void*>(flags_)})
+  for (void* p : {static_cast<void*>(panel_), static_cast<void*>(panel_inv_), static_cast<void*>(panel_invt_),
+                  static_cast<void*>(work_), static_cast<void*>(flags_)})
     if (p) (void)hipFree(p);
 }
 
 template <typename T>
 int DenseCholesky<T>::reserve(int m) {
   if (m <= m_ && flags_) return SOV_OK;
-  for (void* p : {static_cast<void*>(panel_), static_cast<void*>(panel_inv_), static_cast<void*>(work_),
-                  static_cast<void*>(flags_)})
+  for (void* p : {static_cast<void*>(panel_), static_cast<void*>(panel_inv_), static_cast<void*>(panel_invt_),
+                  static_cast<void*>(work_), static_cast<void*>(flags_)})
     if (p) (void)hipFree(p);
-  panel_ = panel_inv_ = nullptr;
+  panel_ = panel_inv_ = panel_invt_ = nullptr;
   work_ = nullptr;
   flags_ = nullptr;
   const int np = (m + kPanel - 1) / kPanel;
   SOV_HIP_TRY(hipMalloc(&panel_, sizeof(T) * static_cast<std::size_t>(m) * kPanel * 2));
   SOV_HIP_TRY(hipMalloc(&panel_inv_, sizeof(T) * static_cast<std::size_t>(np) * kPanel * kPanel));
+  SOV_HIP_TRY(hipMalloc(&panel_invt_, sizeof(T) * static_cast<std::size_t>(np) * kPanel * kPanel));
   SOV_HIP_TRY(hipMalloc(&work_, sizeof(double) * static_cast<std::size_t>(m)));
   SOV_HIP_TRY(hipMalloc(&flags_, sizeof(FactorFlags)));
   SOV_HIP_TRY(hipMemset(flags_, 0, sizeof(FactorFlags)));
@@ -337,7 +438,8 @@ int DenseCholesky<T>::factor(T* a, int m, const double* diag0, hipStream_t s) {
     T* pinv = panel_inv_ + static_cast<long long>(p) * kPanel * kPanel;
     T* L21 = panel_ + (p & 1) * pstride;
     TimedPair* tp = timed_begin(s, kTimePanel);
-    potrf_panel_kernel<T><<<1, 1024, panel_lds_bytes<T>(), s>>>(a, lda, k0, nb, diag0, flags_, pinv);
+    potrf_panel_kernel<T><<<1, kPanelThreads, panel_lds_bytes<T>(), s>>>(a, lda, k0, nb, diag0, flags_, pinv,
+                                                               panel_invt_ + static_cast<long long>(p) * kPanel * kPanel);
     SOV_LAUNCH_CHECK();
     if (below > 0) {
       // L21 = A21 inv(L11)^T on MFMA, out of place, then back into the factor.
@@ -389,7 +491,7 @@ int DenseCholesky<T>::solve(const T* a, int m, double* x, double* /*work*/, hipS
   SOV_LAUNCH_CHECK();
   for (int p = np - 1; p >= 0; --p) {  // L^T x = y: x back into x, work_ updated in place
     const int above = p * kPanel;
-    trsv_bwd_step_kernel<T><<<std::max(1u, blocks_for(above, kSolveRows)), 256, 0, s>>>(a, m, m, p, panel_inv_, work_,
+    trsv_bwd_step_kernel<T><<<std::max(1u, blocks_for(above, kSolveRows)), 256, 0, s>>>(a, m, m, p, panel_invt_, work_,
                                                                                          x);
   }
   SOV_LAUNCH_CHECK();

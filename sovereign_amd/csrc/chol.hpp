@@ -45,7 +45,8 @@ class DenseCholesky {
   T* panel_ = nullptr;      // L21 of two consecutive panels (2 x m x kPanel, leading dimension m)
   hipStream_t side_ = nullptr;          // trailing updates run here, overlapping the next panel
   hipEvent_t ev_panel_ = nullptr, ev_bulk_ = nullptr;
-  T* panel_inv_ = nullptr;  // inverses of all diagonal blocks (kPanel x kPanel each)
+  T* panel_inv_ = nullptr;   // inverses of all diagonal blocks (kPanel x kPanel each)
+  T* panel_invt_ = nullptr;  // their transposes
   double* work_ = nullptr;  // intermediate vector of the solves (m)
   FactorFlags* flags_ = nullptr;
 };

@@ -188,11 +188,12 @@ __global__ void max_finish_kernel(int nparts, const double* __restrict__ partial
   if (threadIdx.x == 0) sc->rnorm = r;
 }
 
-// p = z + beta p (beta = 0 on the first pass via `first`).
+// p = z + beta p; p = z on the first pass (p is not read then: it may hold
+// stale bits, and 0 * NaN would not be 0).
 __global__ __launch_bounds__(kRed) void direction_kernel(int m, const PcgScalars* __restrict__ sc, int first,
                                                          const double* __restrict__ z, double* __restrict__ p) {
   const double beta = first ? 0.0 : sc->beta;
-  for (int i = blockIdx.x * kRed + threadIdx.x; i < m; i += gridDim.x * kRed) p[i] = fma(beta, p[i], z[i]);
+  for (int i = blockIdx.x * kRed + threadIdx.x; i < m; i += gridDim.x * kRed) p[i] = first ? z[i] : fma(beta, p[i], z[i]);
 }
 
 unsigned red_blocks(int m) { return std::max(1u, std::min<unsigned>(kRedBlocks, blocks_for(m, kRed))); }
@@ -404,7 +405,8 @@ int NormalSolver::solve(double* x, hipStream_t s, double tol, int max_iter) {
   double ratio = 0.0;
   if ((rc = check(&ratio)) != SOV_OK) return rc;
   last_res_ = ratio;
-  if (!(ratio > tol)) return SOV_OK;
+  if (!std::isfinite(ratio)) return SOV_SINGULAR;
+  if (ratio <= tol) return SOV_OK;
   SOV_HIP_TRY(hipMemcpyAsync(z_, r_, bytes, hipMemcpyDeviceToDevice, s));
   if ((rc = ch32.solve(M32_, m_, z_, nullptr, s)) != SOV_OK) return rc;
   direction_kernel<<<rb, kRed, 0, s>>>(m_, sc_, 1, z_, p_);
@@ -421,8 +423,8 @@ int NormalSolver::solve(double* x, hipStream_t s, double tol, int max_iter) {
     if ((rc = check(&ratio)) != SOV_OK) return rc;
     last_iters_ = it;
     last_res_ = ratio;
-    if (!(ratio > tol)) return SOV_OK;
     if (!std::isfinite(ratio)) break;
+    if (ratio <= tol) return SOV_OK;
     SOV_HIP_TRY(hipMemcpyAsync(z_, r_, bytes, hipMemcpyDeviceToDevice, s));
     if ((rc = ch32.solve(M32_, m_, z_, nullptr, s)) != SOV_OK) return rc;
     dot_kernel<<<rb, kRed, 0, s>>>(m_, r_, z_, partial_);

@@ -68,9 +68,10 @@ __global__ __launch_bounds__(256) void trsv_fwd_step_kernel(const T* __restrict_
 }
 
 // Backward step p: out[P] = inv(L_pp)^T c[P]; c[above] -= L[P, above]^T out[P].
+// pinvt holds the transposed inverses, so the product reads them coalesced.
 template <typename T>
 __global__ __launch_bounds__(256) void trsv_bwd_step_kernel(const T* __restrict__ a, long long lda, int m, int p,
-                                                            const T* __restrict__ pinv, double* __restrict__ c,
+                                                            const T* __restrict__ pinvt, double* __restrict__ c,
                                                             double* __restrict__ out) {
   __shared__ double cp[kPanel];
   __shared__ double part[2][kPanel];
@@ -78,12 +79,12 @@ __global__ __launch_bounds__(256) void trsv_bwd_step_kernel(const T* __restrict_
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const int base = p * kPanel;
   const int nbp = min(kPanel, m - base);
-  // x = inv^T cp : output k = sum_r inv[r + k*kPanel] cp[r]; thread (k, half)
+  // x_k = sum_r inv(r, k) cp_r = sum_r invt(k, r) cp_r; thread (k, half)
   const int k = tid & (kPanel - 1), half = tid >> 7;
-  const T* inv = pinv + static_cast<long long>(p) * kPanel * kPanel + static_cast<long long>(k) * kPanel + half * 64;
+  const T* invt = pinvt + static_cast<long long>(p) * kPanel * kPanel;
   T iv[64];
 #pragma unroll
-  for (int q = 0; q < 64; ++q) iv[q] = inv[q];
+  for (int q = 0; q < 64; ++q) iv[q] = invt[k + (half * 64 + q) * kPanel];
   // columns above: 64 per workgroup, wavefront w takes 16 of them; lanes run
   // down the block's rows (two 64-row halves) of each column.
   const int col0 = blockIdx.x * kSolveRows + w * 16;

@@ -27,13 +27,15 @@ def rel(a, b):
 
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
 @pytest.mark.parametrize("key", KEYS)
-def test_ipm_matches_reference_golden(gpu_api, key, precision):
+def test_ipm_matches_reference_golden(gpu_api, key, precision, monkeypatch):
     exp = G.expected()[key]
     model = G.model_for(key)
     say(f"ipm {key} {precision}")
+    monkeypatch.setenv("SOVEREIGN_IPM_TRACE", "1")  # per-iteration log on stderr, shown on failure
     r = gpu_api.InteriorPointSolver(gpu_api.InteriorPointOptions(precision=precision)).solve(model)
+    summary = {k: v for k, v in r.items() if not isinstance(v, np.ndarray)}
     assert r["rc"] == 0, r.get("message")
-    assert r["status"] == exp["status"]
+    assert r["status"] == exp["status"], summary
     assert rel(r["objective"], float(exp["objective"])) <= 1e-8
     assert r["primal_residual"] <= 1e-8 and r["dual_residual"] <= 1e-8 and r["duality_gap"] <= 1e-8
     # the primal point satisfies the original constraints
