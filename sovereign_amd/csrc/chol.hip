@@ -35,20 +35,22 @@ constexpr int kPanelLd = kPanel + 1;  // LDS leading dimension of a panel block
 // leading dimension kPanelLd): L in the lower triangle, X^T in the strict
 // upper triangle (X(i, k) at S[i * kPanelLd + k] for i > k) and the diagonal
 // of X in xdiag. It is processed in sub-panels of kSub = 32 columns:
-//   A  wavefront 0 factors the 32 x 32 diagonal block with no workgroup
-//      barrier: lanes 0-31 each hold a row of the block, lanes 32-63 each a
-//      column of its inverse, and both halves run the same right-looking
-//      update against the newly finished column of L, read from LDS;
+//   A  wavefront 0 factors the 32 x 32 diagonal block in registers, with no
+//      workgroup barrier: lanes 0-31 each hold a row of the block, lanes
+//      32-63 each a column of its inverse, and both halves run the same
+//      right-looking update against the newly finished column of L, which
+//      is broadcast from the lanes that own it;
 //   B  rows below it inside the block: L(r, sub) = A(r, sub) inv(L_ss)^T;
-//   C  the rank-32 update of the trailing lower part of the block;
 //   Y/X the inverse blocks left of the diagonal in this block row,
-//      X(t, <t) = -inv(L_tt) (L(t, <t) X(<t, <t)).
+//      X(t, <t) = -inv(L_tt) (L(t, <t) X(<t, <t));
+//   C  the rank-32 update of the trailing lower part of the block.
+// B, Y/X and C are 16 x 16 MFMA tiles with operands read from LDS.
 // The pivot rule is the reference LDL^T's: a pivot <= 1e-13 x |its diagonal
 // before elimination| is dropped (L_jj = DropPivot), a non-finite or
-// < 1e-30 pivot is a breakdown. L11 goes back to `a`, X to inv_out and X^T to
-// invt_out (leading dimension kPanel, zero outside the nb x nb lower
-// triangle); they make the panel TRSM a GEMM and the triangular solves
-// matrix-vector products.
+// < 1e-30 pivot is a breakdown. X goes to inv_out (leading dimension ldi,
+// zero outside the nb x nb lower triangle): it makes the panel TRSM a GEMM
+// and seeds the super-block inverses of the triangular solves. L11 itself is
+// not written back: nothing reads it.
 // ---------------------------------------------------------------------------
 constexpr int kSub = 32;
 constexpr int kPanelThreads = 512;
@@ -68,11 +70,27 @@ __device__ __forceinline__ float read_lane<float>(float v, int lane) {
   return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
 }
 
+// The value of v in lane `lane` into a VGPR of every lane (ds_bpermute: no
+// LDS memory, many can be in flight).
+template <typename T>
+__device__ __forceinline__ T bcast_lane(T v, int lane);
+template <>
+__device__ __forceinline__ double bcast_lane<double>(double v, int lane) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_ds_bpermute(4 * lane, static_cast<int>(b));
+  const int hi = __builtin_amdgcn_ds_bpermute(4 * lane, static_cast<int>(b >> 32));
+  return __longlong_as_double((static_cast<long long>(hi) << 32) | static_cast<unsigned int>(lo));
+}
+template <>
+__device__ __forceinline__ float bcast_lane<float>(float v, int lane) {
+  return __int_as_float(__builtin_amdgcn_ds_bpermute(4 * lane, __float_as_int(v)));
+}
+
 template <typename T>
 __global__ __launch_bounds__(kPanelThreads) void potrf_panel_kernel(T* __restrict__ a, long long lda, int k0, int nb,
                                                                     const double* __restrict__ diag0,
                                                                     FactorFlags* __restrict__ flags,
-                                                                    T* __restrict__ inv_out, T* __restrict__ invt_out) {
+                                                                    T* __restrict__ inv_out, long long ldi) {
   typedef typename Mfma<T>::acc_t acc_t;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   T* S = reinterpret_cast<T*>(smem_raw);                   // [kPanel][kPanelLd]
@@ -132,12 +150,15 @@ __global__ __launch_bounds__(kPanelThreads) void potrf_panel_kernel(T* __restric
           }
           // factor lanes below j: l = a * rpiv; lane j: the pivot; inverse
           // lanes: x_j *= rpiv. Then reg[k] -= reg[j] L(k, j) for k > j,
-          // L(k, j) read from lane k.
+          // L(k, j) broadcast from lane k.
           T v = reg[j] * rpiv;
           if (fac && me == j) v = piv;
           if (!fac || me >= j) reg[j] = v;
+          T col[kSub];
 #pragma unroll
-          for (int k = j + 1; k < kSub; ++k) reg[k] -= reg[j] * read_lane(reg[j], k);
+          for (int k = j + 1; k < kSub; ++k) col[k] = bcast_lane(reg[j], k);
+#pragma unroll
+          for (int k = j + 1; k < kSub; ++k) reg[k] -= reg[j] * col[k];
         }
       }
       if (fac) {  // row me of L
@@ -252,18 +273,15 @@ __global__ __launch_bounds__(kPanelThreads) void potrf_panel_kernel(T* __restric
     __syncthreads();
   }
 
-  // L11 to the factor, X and X^T to the inverse buffers.
+  // X to the inverse buffer, zero outside the nb x nb lower triangle. (L11
+  // itself is not stored back: the solves and the panel TRSM use only X.)
   {
     const int i = tid & (kPanel - 1), k1 = tid >> 7;
 #pragma unroll 8
     for (int q = 0; q < kPanel / 4; ++q) {
       const int k = k1 + 4 * q;
       const bool in = i < nb && k < nb;
-      if (in && i >= k) a[(k0 + i) + static_cast<long long>(k0 + k) * lda] = S[k * kPanelLd + i];
-      const T x = in ? (i > k ? S[i * kPanelLd + k] : (i == k ? xdiag[i] : T(0))) : T(0);
-      const T y = in ? (k > i ? S[k * kPanelLd + i] : (k == i ? xdiag[i] : T(0))) : T(0);
-      inv_out[i + k * kPanel] = x;
-      invt_out[i + k * kPanel] = y;
+      inv_out[i + k * ldi] = in ? (i > k ? S[i * kPanelLd + k] : (i == k ? xdiag[i] : T(0))) : T(0);
     }
   }
 }
@@ -387,7 +405,7 @@ DenseCholesky<T>::~DenseCholesky() {
   if (ev_panel_) (void)hipEventDestroy(ev_panel_);
   if (ev_bulk_) (void)hipEventDestroy(ev_bulk_);
   if (side_) (void)hipStreamDestroy(side_);
-  for (void* p : {static_cast<void*>(panel_), static_cast<void*>(panel_inv_), static_cast<void*>(panel_invt_),
+  for (void* p : {static_cast<void*>(panel_), static_cast<void*>(sup_inv_), static_cast<void*>(tmp_),
                   static_cast<void*>(work_), static_cast<void*>(flags_)})
     if (p) (void)hipFree(p);
 }
@@ -395,16 +413,19 @@ DenseCholesky<T>::~DenseCholesky() {
 template <typename T>
 int DenseCholesky<T>::reserve(int m) {
   if (m <= m_ && flags_) return SOV_OK;
-  for (void* p : {static_cast<void*>(panel_), static_cast<void*>(panel_inv_), static_cast<void*>(panel_invt_),
+  for (void* p : {static_cast<void*>(panel_), static_cast<void*>(sup_inv_), static_cast<void*>(tmp_),
                   static_cast<void*>(work_), static_cast<void*>(flags_)})
     if (p) (void)hipFree(p);
-  panel_ = panel_inv_ = panel_invt_ = nullptr;
+  panel_ = sup_inv_ = tmp_ = nullptr;
   work_ = nullptr;
   flags_ = nullptr;
-  const int np = (m + kPanel - 1) / kPanel;
   SOV_HIP_TRY(hipMalloc(&panel_, sizeof(T) * static_cast<std::size_t>(m) * kPanel * 2));
-  SOV_HIP_TRY(hipMalloc(&panel_inv_, sizeof(T) * static_cast<std::size_t>(np) * kPanel * kPanel));
-  SOV_HIP_TRY(hipMalloc(&panel_invt_, sizeof(T) * static_cast<std::size_t>(np) * kPanel * kPanel));
+  const std::size_t nsb = static_cast<std::size_t>((m + kSuper - 1) / kSuper);
+  const std::size_t sq = static_cast<std::size_t>(kSuper) * kSuper;
+  // Blocks above the diagonal of each super-block stay zero: nothing writes them.
+  SOV_HIP_TRY(hipMalloc(&sup_inv_, sizeof(T) * nsb * sq));
+  SOV_HIP_TRY(hipMemset(sup_inv_, 0, sizeof(T) * nsb * sq));
+  SOV_HIP_TRY(hipMalloc(&tmp_, sizeof(T) * nsb * sq / 4));
   SOV_HIP_TRY(hipMalloc(&work_, sizeof(double) * static_cast<std::size_t>(m)));
   SOV_HIP_TRY(hipMalloc(&flags_, sizeof(FactorFlags)));
   SOV_HIP_TRY(hipMemset(flags_, 0, sizeof(FactorFlags)));
@@ -435,16 +456,17 @@ int DenseCholesky<T>::factor(T* a, int m, const double* diag0, hipStream_t s) {
   for (int k0 = 0, p = 0; k0 < m; k0 += kPanel, ++p) {
     const int nb = std::min(kPanel, m - k0);
     const int below = m - k0 - nb;
-    T* pinv = panel_inv_ + static_cast<long long>(p) * kPanel * kPanel;
+    // the panel inverse is the diagonal block p % 8 of super-block p / 8
+    T* pinv = sup_inv_ + static_cast<long long>(k0 / kSuper) * kSuper * kSuper +
+              static_cast<long long>(k0 % kSuper) * (kSuper + 1);
     T* L21 = panel_ + (p & 1) * pstride;
     TimedPair* tp = timed_begin(s, kTimePanel);
-    potrf_panel_kernel<T><<<1, kPanelThreads, panel_lds_bytes<T>(), s>>>(a, lda, k0, nb, diag0, flags_, pinv,
-                                                               panel_invt_ + static_cast<long long>(p) * kPanel * kPanel);
+    potrf_panel_kernel<T><<<1, kPanelThreads, panel_lds_bytes<T>(), s>>>(a, lda, k0, nb, diag0, flags_, pinv, kSuper);
     SOV_LAUNCH_CHECK();
     if (below > 0) {
       // L21 = A21 inv(L11)^T on MFMA, out of place, then back into the factor.
       const T* a21 = a + (k0 + nb) + static_cast<long long>(k0) * lda;
-      SOV_HIP_TRY((gemm_update<T, false, false, true>(below, nb, nb, a21, lda, pinv, kPanel, L21, lda, T(1), s)));
+      SOV_HIP_TRY((gemm_update<T, false, false, true>(below, nb, nb, a21, lda, pinv, kSuper, L21, lda, T(1), s)));
       copy_panel_kernel<T><<<blocks_for(static_cast<long long>(below) * nb, 256), 256, 0, s>>>(L21, a, lda, k0 + nb,
                                                                                                k0, below, nb);
       SOV_LAUNCH_CHECK();
@@ -475,27 +497,86 @@ int DenseCholesky<T>::factor(T* a, int m, const double* diag0, hipStream_t s) {
   }
   SOV_HIP_TRY(hipEventRecord(ev_bulk_, side_));
   SOV_HIP_TRY(hipStreamWaitEvent(s, ev_bulk_, 0));
+  return build_super_inverses(a, m, s);
+}
+
+// Inverse of each kSuper x kSuper diagonal block of L from its panel
+// inverses, by recursive doubling: for adjacent diagonal blocks A (size b,
+// inverse X_A) and B below it (inverse X_B), the block of the inverse under
+// X_A is X_BA = -X_B (L_BA X_A). One batched GEMM pair per level covers all
+// full super-blocks; a partial last super-block gets its own launches.
+template <typename T>
+int DenseCholesky<T>::build_super_inverses(const T* a, int m, hipStream_t s) {
+  const long long lda = m, ks = kSuper, sq = ks * ks;
+  const int nfull = m / kSuper, rem = m - nfull * kSuper;
+  TimedPair* tp = timed_begin(s, kTimePanel);
+  int launches = 0;
+  for (long long b = kPanel; b < kSuper; b *= 2) {
+    const int npairs = static_cast<int>(kSuper / (2 * b));
+    if (nfull > 0) {
+      GemmBatch g1;  // tmp = L_BA X_A
+      g1.a_y = 2 * b * (1 + lda);
+      g1.a_z = ks * (1 + lda);
+      g1.b_y = 2 * b * (1 + ks);
+      g1.b_z = sq;
+      g1.c_y = b * b;
+      g1.c_z = npairs * b * b;
+      SOV_HIP_TRY((gemm_update<T, true, false, true>(static_cast<int>(b), static_cast<int>(b), static_cast<int>(b),
+                                                     a + b, lda, sup_inv_, ks, tmp_, b, T(1), s, npairs, nfull, g1)));
+      GemmBatch g2;  // X_BA = -X_B tmp
+      g2.a_y = 2 * b * (1 + ks);
+      g2.a_z = sq;
+      g2.b_y = b * b;
+      g2.b_z = npairs * b * b;
+      g2.c_y = 2 * b * (1 + ks);
+      g2.c_z = sq;
+      SOV_HIP_TRY((gemm_update<T, true, false, true>(static_cast<int>(b), static_cast<int>(b), static_cast<int>(b),
+                                                     sup_inv_ + b * (1 + ks), ks, tmp_, b, sup_inv_ + b, ks, T(-1), s,
+                                                     npairs, nfull, g2)));
+      launches += 2;
+    }
+    if (rem > 0) {
+      const long long g0 = static_cast<long long>(nfull) * kSuper;
+      T* X = sup_inv_ + nfull * sq;
+      for (long long i0 = 0; i0 + b < rem; i0 += 2 * b) {
+        const int bb = static_cast<int>(std::min(b, rem - i0 - b));
+        const T* lba = a + (g0 + i0 + b) + (g0 + i0) * lda;
+        SOV_HIP_TRY((gemm_update<T, true, false, true>(bb, static_cast<int>(b), static_cast<int>(b), lba, lda,
+                                                       X + i0 * (1 + ks), ks, tmp_, bb, T(1), s)));
+        SOV_HIP_TRY((gemm_update<T, true, false, true>(bb, static_cast<int>(b), bb, X + (i0 + b) * (1 + ks), ks, tmp_,
+                                                       bb, X + (i0 + b) + i0 * ks, ks, T(-1), s)));
+        launches += 2;
+      }
+    }
+  }
+  timed_end(tp, s, launches, 0.0);
   return SOV_OK;
 }
 
 template <typename T>
 int DenseCholesky<T>::solve(const T* a, int m, double* x, double* /*work*/, hipStream_t s) {
   if (m > m_) return SOV_ERR_ARG;
-  const int np = (m + kPanel - 1) / kPanel;
+  const long long lda = m, sq = static_cast<long long>(kSuper) * kSuper;
+  const int nsb = (m + kSuper - 1) / kSuper;
   TimedPair* tp = timed_begin(s, kTimeSolve);
-  for (int p = 0; p < np; ++p) {  // L y = x: y into work_, x updated in place
-    const int below = std::max(0, m - (p + 1) * kPanel);
-    trsv_fwd_step_kernel<T><<<std::max(1u, blocks_for(below, kSolveRows)), 256, 0, s>>>(a, m, m, p, panel_inv_, x,
-                                                                                         work_);
+  for (int P = 0; P < nsb; ++P) {  // L y = x: y into work_, x updated in place
+    const int base = P * kSuper, sz = std::min(kSuper, m - base), below = m - base - sz;
+    gemv_n_kernel<T, false, true><<<blocks_for(sz, kGemvRows), 256, 0, s>>>(sz, sz, sup_inv_ + P * sq, kSuper,
+                                                                             x + base, work_ + base);
+    if (below > 0)
+      gemv_n_kernel<T, true, false><<<blocks_for(below, kGemvRows), 256, 0, s>>>(
+          below, sz, a + (base + sz) + base * lda, lda, work_ + base, x + base + sz);
   }
   SOV_LAUNCH_CHECK();
-  for (int p = np - 1; p >= 0; --p) {  // L^T x = y: x back into x, work_ updated in place
-    const int above = p * kPanel;
-    trsv_bwd_step_kernel<T><<<std::max(1u, blocks_for(above, kSolveRows)), 256, 0, s>>>(a, m, m, p, panel_invt_, work_,
-                                                                                         x);
+  for (int P = nsb - 1; P >= 0; --P) {  // L^T x = y: x back into x, work_ updated in place
+    const int base = P * kSuper, sz = std::min(kSuper, m - base);
+    gemv_t_kernel<T, false, true><<<blocks_for(sz, 4), 256, 0, s>>>(sz, sz, sup_inv_ + P * sq, kSuper, work_ + base,
+                                                                      x + base);
+    if (base > 0)
+      gemv_t_kernel<T, true, false><<<blocks_for(base, 4), 256, 0, s>>>(sz, base, a + base, lda, x + base, work_);
   }
   SOV_LAUNCH_CHECK();
-  timed_end(tp, s, 2 * np, 0.0);
+  timed_end(tp, s, 4 * nsb, 0.0);
   return SOV_OK;
 }
 

@@ -18,7 +18,8 @@ struct DropPivot<float> {
   static constexpr float value = 1e18f;
 };
 
-constexpr int kPanel = 128;  // factorization panel width (K of the trailing update, solve block)
+constexpr int kPanel = 128;   // factorization panel width (K of the trailing update)
+constexpr int kSuper = 1024;  // diagonal blocks whose inverses the triangular solves use
 
 // Per-factorization device state shared by the kernels.
 struct FactorFlags {
@@ -41,12 +42,14 @@ class DenseCholesky {
   FactorFlags* flags() const { return flags_; }
 
  private:
+  int build_super_inverses(const T* a, int m, hipStream_t s);
+
   int m_ = 0;
   T* panel_ = nullptr;      // L21 of two consecutive panels (2 x m x kPanel, leading dimension m)
   hipStream_t side_ = nullptr;          // trailing updates run here, overlapping the next panel
   hipEvent_t ev_panel_ = nullptr, ev_bulk_ = nullptr;
-  T* panel_inv_ = nullptr;   // inverses of all diagonal blocks (kPanel x kPanel each)
-  T* panel_invt_ = nullptr;  // their transposes
+  T* sup_inv_ = nullptr;  // inverses of the kSuper x kSuper diagonal blocks of L (leading dimension kSuper)
+  T* tmp_ = nullptr;      // products of the recursive doubling that builds them
   double* work_ = nullptr;  // intermediate vector of the solves (m)
   FactorFlags* flags_ = nullptr;
 };

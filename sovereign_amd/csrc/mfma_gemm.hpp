@@ -61,6 +61,11 @@ __device__ __forceinline__ int xcd_swizzle(int b, int grid) {
   return (b & 7) * (full >> 3) + (b >> 3);
 }
 
+// Element offsets of the operands of batch entry (blockIdx.y, blockIdx.z).
+struct GemmBatch {
+  long long a_y = 0, a_z = 0, b_y = 0, b_z = 0, c_y = 0, c_z = 0;
+};
+
 // Tile coordinates of linear tile t: lower triangle (ti >= tj) or full.
 __device__ __forceinline__ void tile_coords(int t, bool lower, int tiles_m, int* ti, int* tj) {
   if (lower) {
@@ -79,8 +84,11 @@ template <typename T, bool NN, bool LOWER, bool STORE = false>
 __global__ __launch_bounds__(256) void gemm_update_kernel(int M, int N, int K, const T* __restrict__ A,
                                                           long long lda, const T* __restrict__ B,
                                                           long long ldb, T* __restrict__ C,
-                                                          long long ldc, T alpha) {
+                                                          long long ldc, T alpha, GemmBatch bt) {
   typedef typename Mfma<T>::acc_t acc_t;
+  A += blockIdx.y * bt.a_y + blockIdx.z * bt.a_z;
+  B += blockIdx.y * bt.b_y + blockIdx.z * bt.b_z;
+  C += blockIdx.y * bt.c_y + blockIdx.z * bt.c_z;
   __shared__ T As[kChunk * kLdsStride];
   __shared__ T Bs[kChunk * kLdsStride];
 
@@ -168,15 +176,17 @@ __global__ __launch_bounds__(256) void gemm_update_kernel(int M, int N, int K, c
   }
 }
 
-// Host launcher. Returns a hipError_t from the launch.
+// Host launcher: one product, or ny x nz of them offset by `bt`. Returns a
+// hipError_t from the launch.
 template <typename T, bool NN, bool LOWER, bool STORE = false>
 inline hipError_t gemm_update(int M, int N, int K, const T* A, long long lda, const T* B, long long ldb, T* C,
-                              long long ldc, T alpha, hipStream_t s) {
-  if (M <= 0 || N <= 0 || K <= 0) return hipSuccess;
+                              long long ldc, T alpha, hipStream_t s, int ny = 1, int nz = 1,
+                              GemmBatch bt = GemmBatch{}) {
+  if (M <= 0 || N <= 0 || K <= 0 || ny <= 0 || nz <= 0) return hipSuccess;
   const long long tm = (M + kTile - 1) / kTile, tn = (N + kTile - 1) / kTile;
   const long long tiles = LOWER ? tm * (tm + 1) / 2 : tm * tn;
-  gemm_update_kernel<T, NN, LOWER, STORE><<<static_cast<unsigned>(tiles), 256, 0, s>>>(M, N, K, A, lda, B, ldb, C, ldc,
-                                                                                     alpha);
+  const dim3 grid(static_cast<unsigned>(tiles), static_cast<unsigned>(ny), static_cast<unsigned>(nz));
+  gemm_update_kernel<T, NN, LOWER, STORE><<<grid, 256, 0, s>>>(M, N, K, A, lda, B, ldb, C, ldc, alpha, bt);
   return hipGetLastError();
 }
 
