@@ -1,30 +1,41 @@
-"""Benchmark: interior-point solve time on the m=8192 random sparse LP.
+"""Benchmark: interior-point solve time on the m=8192 random sparse LP, and
+strong-branching throughput on the n=500, m=200 multidimensional knapsack.
 
-One step = one full Mehrotra IPM solve to a 1e-8 relative gap (and 1e-8
+IPM step = one full Mehrotra IPM solve to a 1e-8 relative gap (and 1e-8
 primal/dual residuals) of the LP
   m = 8192 rows, n = 32768 columns, 6 nonzeros per column at uniform rows,
-  values N(0,1), b = A x0, c = A'y0 + s0 (seed 26119),
+  values N(0,1), b = A x0 with x0 ~ U(0.1, 2), c = A'y0 + s0 (seed 26119),
 with the matrix already resident in HBM (uploaded and analysed once before the
 timed region). `value` is the mixed-precision GPU path (FP32 MFMA
 factorizations while mu >= 1e-6, FP64 residuals and PCG refinement); the plain
 FP64 GPU path is timed in the same run and reported beside it.
 
-Roofline: the dominant kernel is the MFMA trailing update (SYRK) of the
-blocked Cholesky; its algorithmic flops (lower triangle, M(M+1)K per launch)
-over its HIP-event-timed duration during the timed region.
+Strong branching: the root LP of the knapsack (500 binaries, 200 <= rows,
+seed 26119) is solved, then its 64 most fractional candidates are scored with
+2 x 64 = 128 child LPs, once in one persistent batched launch and once with
+the plain per-LP GPU interior-point path; both must give the same scores.
+
+Roofline: the dominant MFMA kernel of the measured (mixed) path, the bulk
+trailing update (SYRK) of the blocked Cholesky, with its algorithmic flops
+(lower triangle, M(M+1)K per launch) over its HIP-event-timed duration (events
+on the stream the kernel runs on) during the timed region. Its HBM traffic
+per launch comes from rocprofv3 FETCH_SIZE / WRITE_SIZE passes recorded in
+profiles/traffic.json (null when absent).
 
 CPU baseline: the oracle's restatement of the reference CPU interior-point
-path (minimum-degree ordered sparse LDL^T, single thread) on a bounded sample
-of the same generator, extrapolated to m = 8192 by the cube of the size ratio
-(the factorization dominates and its fill is near-dense on this structure).
+path (minimum-degree ordered sparse LDL^T, one thread) on two bounded
+samples of the same generator (m = 1024 and 2048), extrapolated to m = 8192
+with the exponent fitted between them.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+                       [--paths mixed,fp64] [--no-branching]
 """
 from __future__ import annotations
 
 import argparse
 import ctypes as C
 import json
+import math
 import os
 import sys
 import time
@@ -36,19 +47,27 @@ import numpy as np  # noqa: E402
 
 METRIC = "IPM solve s & strong-branch LP solves/s at 1e-8 gap vs plain FP64 GPU path"
 MFMA_PEAK = {"f64": 78.6, "f32": 157.3}  # TFLOP/s, MI355X dense matrix peaks
+KERNEL_NAME = {"f64": "gemm_update_kernel<double, false, true, false>",
+               "f32": "gemm_update_kernel<float, false, true, false>"}
+NT = 12  # slots of sov_kernel_times
 
 
 def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
+def kernel_times(lib):
+    t = np.zeros(NT)
+    lib.sov_kernel_times(t.ctypes.data_as(C.POINTER(C.c_double)))
+    return t
+
+
 def timed_solves(ipm, opts, steps, warmup, lib):
-    for w in range(warmup):
+    for _ in range(warmup):
         r = ipm.solve(opts)
         log(f"warmup {opts.precision}: {r['status_name']} it={r['iterations']} {r['seconds']:.3f}s")
     lib.sov_sync()
-    times = np.zeros(7)
-    lib.sov_kernel_times(times.ctypes.data_as(C.POINTER(C.c_double)))  # reset
+    kernel_times(lib)  # reset
     lib.sov_set_kernel_timing(1)
     out = []
     t0 = time.perf_counter()
@@ -61,26 +80,90 @@ def timed_solves(ipm, opts, steps, warmup, lib):
     lib.sov_sync()
     wall = time.perf_counter() - t0
     lib.sov_set_kernel_timing(0)
-    lib.sov_kernel_times(times.ctypes.data_as(C.POINTER(C.c_double)))
-    return out, wall, times
+    return out, wall, kernel_times(lib)
 
 
-def cpu_baseline(sample_m: int):
+def kernel_ms(kt):
+    return {"syrk_f64": kt[0], "syrk_f32": kt[9], "panel_chain": kt[3], "strip_update": kt[7],
+            "super_inverse": kt[8], "triangular_solves": kt[4], "assembly": kt[5]}
+
+
+def load_traffic():
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    if not os.path.exists(path):
+        return {}
+    with open(path) as fh:
+        return json.load(fh)
+
+
+def roofline(kt, dtype, traffic):
+    ms, launches, flops = (kt[0], kt[1], kt[2]) if dtype == "f64" else (kt[9], kt[10], kt[11])
+    if ms <= 0 or launches <= 0:
+        return None
+    achieved = flops / (ms * 1e-3) / 1e12
+    tr = traffic.get(KERNEL_NAME[dtype])
+    return {"bound": "mfma", "achieved": achieved, "peak": MFMA_PEAK[dtype], "unit": "TFLOP/s",
+            "frac": achieved / MFMA_PEAK[dtype],
+            "traffic": tr["bytes_per_launch"] if tr else None,
+            "kernel": KERNEL_NAME[dtype] + " (bulk SYRK trailing update)",
+            "launches": int(launches), "avg_launch_us": ms * 1e3 / launches,
+            "flops_per_launch": flops / launches,
+            "traffic_source": tr.get("source") if tr else None}
+
+
+def cpu_baseline(sizes=(1024, 2048)):
     from oracle import oracle as O
     from sovereign_amd import problems as P
 
-    lp = P.random_sparse_lp(sample_m, 4 * sample_m, 6, seed=P.BASE_SEED)
-    t0 = time.perf_counter()
-    r = O.solve_form(lp, "auto", feasibility_tol=1e-8, optimality_tol=1e-8)
-    dt = time.perf_counter() - t0
-    factor = (8192 / sample_m) ** 3
+    runs = []
+    for m in sizes:
+        lp = P.random_sparse_lp(m, 4 * m, 6, seed=P.BASE_SEED)
+        t0 = time.perf_counter()
+        r = O.solve_form(lp, "auto", feasibility_tol=1e-8, optimality_tol=1e-8)
+        runs.append((m, time.perf_counter() - t0, r["iterations"], r["status"]))
+    (m1, t1, _, _), (m2, t2, it2, st2) = runs[0], runs[-1]
+    expo = math.log(t2 / t1) / math.log(m2 / m1)
+    value = t2 * (8192 / m2) ** expo
     return {
-        "value": dt * factor, "unit": "s per IPM solve (extrapolated to m=8192)", "cores": 1, "kind": "port",
-        "sample": (f"oracle restatement of the reference CPU IPM (AMD-ordered sparse LDL^T), full solve of the "
-                   f"same generator at m={sample_m}, n={4 * sample_m}: {dt:.2f} s, {r['iterations']} iterations, "
-                   f"status {r['status']}; x (8192/{sample_m})^3"),
-        "host_cpus": os.cpu_count(),
+        "value": value, "unit": "s per IPM solve (extrapolated to m=8192)", "cores": 1, "kind": "port",
+        "sample": (f"oracle restatement of the reference CPU IPM (AMD-ordered sparse LDL^T, one thread), full "
+                   f"solves of the same generator at m={m1} ({t1:.2f} s) and m={m2} ({t2:.2f} s, {it2} iterations, "
+                   f"status {st2}); time ~ m^{expo:.2f} fitted between them, extrapolated to m=8192"),
     }
+
+
+def branching_leg(lib):
+    """128 strong-branch child LPs of the n=500, m=200 knapsack root: batched vs per-LP."""
+    from sovereign_amd import api, milp, problems as P
+
+    model = P.multi_knapsack(500, 200, seed=P.BASE_SEED)
+    is_int = np.ones(model.n_vars, dtype=np.int8)
+    root = api.InteriorPointSolver(api.InteriorPointOptions(feasibility_tol=1e-9, optimality_tol=1e-9)).solve(model)
+    nfrac = int(np.sum(np.abs(root["primal"] - np.round(root["primal"])) > 1e-6))
+    out = {"model": "multi_knapsack n=500 m=200 seed 26119 (root node)", "fractional_at_root": nfrac}
+    milp.strong_branch(model, is_int, root["primal"], root["objective"], max_candidates=8)  # warm the batch path
+    lib.sov_sync()
+    t0 = time.perf_counter()
+    cb, best_b, kms = milp.strong_branch(model, is_int, root["primal"], root["objective"], max_candidates=64)
+    wall_b = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    cp, best_p, _ = milp.strong_branch(model, is_int, root["primal"], root["objective"], max_candidates=64,
+                                       child_solver=milp.CHILD_PER_LP)
+    wall_p = time.perf_counter() - t0
+    children = 2 * len(cb)
+    diff = max([abs(a[f"obj_{s}"] - b[f"obj_{s}"]) / max(1.0, abs(b[f"obj_{s}"]))
+                for a, b in zip(cb, cp) for s in ("down", "up")] or [0.0])
+    out.update({
+        "candidates": len(cb), "child_lps": children,
+        "batched_wall_s": wall_b, "batched_kernel_ms": kms, "batched_lp_solves_per_s": children / wall_b,
+        "per_lp_wall_s": wall_p, "per_lp_lp_solves_per_s": children / wall_p,
+        "speedup_batched_vs_per_lp": wall_p / wall_b,
+        "same_choice": best_b == best_p, "max_child_objective_rel_diff": diff,
+        "all_children_solved": all(c["status_down"] in (0, 2) and c["status_up"] in (0, 2) for c in cb),
+    })
+    log(f"strong branching: {children} child LPs batched {wall_b:.3f}s ({kms:.1f} ms kernel), per-LP "
+        f"{wall_p:.3f}s, same choice {best_b == best_p}, max rel diff {diff:.2e}")
+    return out
 
 
 def main():
@@ -89,7 +172,8 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-sample-m", type=int, default=2048)
+    ap.add_argument("--no-branching", action="store_true")
+    ap.add_argument("--paths", default="mixed,fp64")
     ap.add_argument("--m", type=int, default=8192)
     ap.add_argument("--n", type=int, default=32768)
     args = ap.parse_args()
@@ -97,6 +181,7 @@ def main():
         print("bench.py measures one GPU: this build has no multi-GPU path (the sharding design is in DESIGN.md).",
               file=sys.stderr)
         sys.exit(2)
+    paths = [p for p in args.paths.split(",") if p]
 
     from sovereign_amd import _capi, api, problems as P
 
@@ -116,64 +201,56 @@ def main():
     log(f"resident setup (upload + normal-equation plan) {setup_s:.3f}s")
 
     tol = dict(feasibility_tol=1e-8, optimality_tol=1e-8)
-    mixed = api.InteriorPointOptions(precision="mixed", **tol)
-    fp64 = api.InteriorPointOptions(precision="fp64", **tol)
-    res_mx, wall_mx, kt_mx = timed_solves(ipm, mixed, args.steps, args.warmup, lib)
-    res_64, wall_64, kt_64 = timed_solves(ipm, fp64, args.steps, args.warmup, lib)
-    all_optimal = all(r["status"] == 0 for r in res_mx + res_64)
-    if not all_optimal:
-        log("WARNING: a timed solve did not reach optimality: "
-            + str([(r["status_name"], r["duality_gap"]) for r in res_mx + res_64]))
-    log(f"kernel ms (mixed run): syrk {kt_mx[0]:.1f} ({int(kt_mx[1])} launches) panel {kt_mx[3]:.1f} "
-        f"solve {kt_mx[4]:.1f}; (fp64 run): syrk {kt_64[0]:.1f} ({int(kt_64[1])}) panel {kt_64[3]:.1f} "
-        f"solve {kt_64[4]:.1f}")
-    sec_mx = wall_mx / args.steps
-    sec_64 = wall_64 / args.steps
-    obj_gap = abs(res_mx[-1]["objective"] - res_64[-1]["objective"]) / max(1.0, abs(res_64[-1]["objective"]))
-
-    # dominant kernel: the trailing-update SYRK with the most time in the mixed run
-    def roof(kt, dtype):
-        ms, launches, flops = kt[0], kt[1], kt[2]
-        if ms <= 0 or launches <= 0:
-            return None
-        achieved = flops / (ms * 1e-3) / 1e12
-        return {"bound": "mfma", "achieved": achieved, "peak": MFMA_PEAK[dtype], "unit": "TFLOP/s",
-                "frac": achieved / MFMA_PEAK[dtype], "traffic": None,
-                "kernel": f"gemm_update_kernel<{'double' if dtype == 'f64' else 'float'}> (SYRK trailing update)",
-                "launches": int(launches), "avg_launch_us": ms * 1e3 / launches,
-                "flops_per_launch": flops / launches}
-
-    roof64 = roof(kt_64, "f64")
+    runs = {}
+    for path in paths:
+        opts = api.InteriorPointOptions(precision=path, **tol)
+        runs[path] = timed_solves(ipm, opts, args.steps, args.warmup, lib)
+        kt = runs[path][2]
+        log(f"kernel ms ({path} run): " + " ".join(f"{k} {v:.1f}" for k, v in kernel_ms(kt).items()))
+    ipm.close()
+    main_path = "mixed" if "mixed" in runs else paths[0]
+    res, wall, kt = runs[main_path]
+    traffic = load_traffic()
+    # dominant MFMA kernel of the measured path: the SYRK precision with the most time
+    dom = "f32" if kt[9] >= kt[0] else "f64"
     line = {
         "metric": METRIC,
-        "value": sec_mx,
+        "value": wall / args.steps,
         "unit": "s per IPM solve (m=8192 LP to 1e-8 gap)",
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-        "ms_per_step": sec_mx * 1e3,
+        "ms_per_step": wall / args.steps * 1e3,
         "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f64 residuals/refinement + f32 factorizations (mu>=1e-6)",
+        "dtype": ("f32 MFMA factorizations while mu >= 1e-6, f64 residuals/PCG/late factorizations"
+                  if main_path == "mixed" else "f64"),
         "data": "synthetic",
         "config": {"workload": f"random sparse LP m={lp.m} n={lp.n} 6 nnz/col seed {P.BASE_SEED}",
-                   "gap_tol": 1e-8, "path": "mixed precision (value); plain FP64 alongside"},
-        "all_optimal": all_optimal,
-        "fp64_path_s": sec_64,
-        "speedup_vs_fp64": sec_64 / sec_mx,
-        "objective": res_64[-1]["objective"], "objective_rel_diff_mixed_vs_fp64": obj_gap,
-        "iterations": {"mixed": res_mx[-1]["iterations"], "fp64": res_64[-1]["iterations"]},
-        "factorizations": {"mixed_fp32": res_mx[-1]["fp32_factorizations"],
-                           "mixed_total": res_mx[-1]["factorizations"], "fp64": res_64[-1]["factorizations"]},
-        "pcg_iterations_mixed": res_mx[-1]["refine_iterations"],
-        "max_solve_residual_mixed": res_mx[-1]["max_solve_residual"],
+                   "gap_tol": 1e-8, "path": f"{main_path} (value)" + (
+                       "; plain FP64 alongside" if "fp64" in runs and main_path != "fp64" else "")},
+        "all_optimal": all(r["status"] == 0 for rr in runs.values() for r in rr[0]),
+        "iterations": {p: rr[0][-1]["iterations"] for p, rr in runs.items()},
+        "factorizations": {p: {"fp32": rr[0][-1]["fp32_factorizations"], "total": rr[0][-1]["factorizations"]}
+                           for p, rr in runs.items()},
         "resident_setup_s": setup_s,
-        "kernel_ms_fp64_run": {"syrk": kt_64[0], "panel": kt_64[3], "solve": kt_64[4]},
-        "kernel_ms_mixed_run": {"syrk": kt_mx[0], "panel": kt_mx[3], "solve": kt_mx[4]},
-        "roofline": roof64,
-        "roofline_f32_syrk": roof(kt_mx, "f32") if res_mx[-1]["fp32_factorizations"] else None,
+        "kernel_ms": {p: kernel_ms(rr[2]) for p, rr in runs.items()},
+        "roofline": roofline(kt, dom, traffic),
     }
+    if "mixed" in runs and "fp64" in runs:
+        s_mx = runs["mixed"][1] / args.steps
+        s_64 = runs["fp64"][1] / args.steps
+        o_mx = runs["mixed"][0][-1]["objective"]
+        o_64 = runs["fp64"][0][-1]["objective"]
+        line.update({
+            "fp64_path_s": s_64, "speedup_vs_fp64": s_64 / s_mx, "objective": o_64,
+            "objective_rel_diff_mixed_vs_fp64": abs(o_mx - o_64) / max(1.0, abs(o_64)),
+            "pcg_iterations_mixed": runs["mixed"][0][-1]["refine_iterations"],
+            "max_solve_residual_mixed": runs["mixed"][0][-1]["max_solve_residual"],
+            "roofline_fp64_path": roofline(runs["fp64"][2], "f64", traffic),
+        })
+    if not args.no_branching:
+        line["strong_branching"] = branching_leg(lib)
     if not args.no_cpu:
-        log(f"cpu baseline sample m={args.cpu_sample_m}")
-        line["cpu_baseline"] = cpu_baseline(args.cpu_sample_m)
-    ipm.close()
+        log("cpu baseline (oracle, m=1024 and m=2048)")
+        line["cpu_baseline"] = cpu_baseline()
     print(json.dumps(line), flush=True)
 
 

@@ -63,14 +63,20 @@ int sov_sync(void);
 void* sov_stream(void);
 
 /* Kernel timing: with timing on, the factorization and solve kernels are
- * bracketed by HIP events on the library stream (this adds host waits).
- * sov_kernel_times fills out[0..6] with the totals since the last call and
- * resets them: trailing-update (SYRK) ms, SYRK launches, SYRK flops
- * (algorithmic, lower triangle), panel (potrf + trsm) ms, triangular-solve
- * ms, assembly ms, panel launches. No counterpart in the reference, which
- * times nothing on the device. */
+ * bracketed by HIP events on the streams they run on (recording does not
+ * wait; the totals are collected after one stream synchronisation).
+ * sov_kernel_times fills out[0..11] with the totals since the last call and
+ * resets them:
+ *   0 FP64 trailing-update SYRK ms (the bulk update of each panel, MFMA),
+ *   1 its launches, 2 its flops (algorithmic: lower triangle, M(M+1)K),
+ *   3 panel chain ms (diagonal-block kernel + panel TRSM + copy),
+ *   4 triangular-solve ms, 5 normal-equation assembly ms,
+ *   6 panel-chain launches, 7 look-ahead strip update ms,
+ *   8 super-block inverse ms,
+ *   9 FP32 trailing-update SYRK ms, 10 its launches, 11 its flops.
+ * No counterpart in the reference, which times nothing on the device. */
 void sov_set_kernel_timing(int32_t on);
-int sov_kernel_times(double* out7);
+int sov_kernel_times(double* out12);
 
 /* -------------------------------------------------------------- dense LU --
  * Replaces cuda::dense_lu (cuda_driver.hpp:29-34; run_dense_lu at

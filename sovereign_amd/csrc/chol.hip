@@ -70,20 +70,20 @@ __device__ __forceinline__ float read_lane<float>(float v, int lane) {
   return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
 }
 
-// The value of v in lane `lane` into a VGPR of every lane (ds_bpermute: no
-// LDS memory, many can be in flight).
+// Each lane receives v from lane `src` (its own choice): ds_bpermute, which
+// goes through the LDS crossbar without touching LDS memory.
 template <typename T>
-__device__ __forceinline__ T bcast_lane(T v, int lane);
+__device__ __forceinline__ T bcast_from(T v, int src);
 template <>
-__device__ __forceinline__ double bcast_lane<double>(double v, int lane) {
+__device__ __forceinline__ double bcast_from<double>(double v, int src) {
   const long long b = __double_as_longlong(v);
-  const int lo = __builtin_amdgcn_ds_bpermute(4 * lane, static_cast<int>(b));
-  const int hi = __builtin_amdgcn_ds_bpermute(4 * lane, static_cast<int>(b >> 32));
+  const int lo = __builtin_amdgcn_ds_bpermute(4 * src, static_cast<int>(b));
+  const int hi = __builtin_amdgcn_ds_bpermute(4 * src, static_cast<int>(b >> 32));
   return __longlong_as_double((static_cast<long long>(hi) << 32) | static_cast<unsigned int>(lo));
 }
 template <>
-__device__ __forceinline__ float bcast_lane<float>(float v, int lane) {
-  return __int_as_float(__builtin_amdgcn_ds_bpermute(4 * lane, __float_as_int(v)));
+__device__ __forceinline__ float bcast_from<float>(float v, int src) {
+  return __int_as_float(__builtin_amdgcn_ds_bpermute(4 * src, __float_as_int(v)));
 }
 
 template <typename T>
@@ -120,20 +120,29 @@ __global__ __launch_bounds__(kPanelThreads) void potrf_panel_kernel(T* __restric
     T* D = S + c * (kPanelLd + 1);     // its diagonal block: element (i, k) at D[k * kPanelLd + i]
 
     // ---- A: diagonal block and its inverse in the registers of wavefront 0.
-    // Lane me < 32 holds row me of the block, lane 32 + me column me of the
-    // inverse; column j of L is broadcast from the lanes that own it.
+    // The 64 lanes form an 8 x 8 grid (p = lane & 7, q = lane >> 3); lane
+    // (p, q) holds the elements (p + 8r, q + 8s), r, s < 4, of the block (f)
+    // and of its inverse X (x). Step j scales column j of L and row j of X,
+    // then applies the rank-1 updates; each lane fetches the four L(i, j) of
+    // its rows, the four L(k, j) and X(j, k) of its columns from the lanes
+    // that own them.
     if (wave == 0) {
-      const bool fac = lane < kSub;
-      const int me = lane & (kSub - 1);
-      T reg[kSub];
+      const int p = lane & 7, q = lane >> 3;
+      T f[4][4], x[4][4];
 #pragma unroll
-      for (int k = 0; k < kSub; ++k)
-        reg[k] = fac ? ((me < sb && k <= me) ? D[k * kPanelLd + me] : T(0)) : (k == me ? T(1) : T(0));
-      const double thr = me < sb ? 1e-13 * fabs(d0[c + me]) : 0.0;  // drop threshold of column me
+      for (int r = 0; r < 4; ++r)
+#pragma unroll
+        for (int s2 = 0; s2 < 4; ++s2) {
+          const int i = p + 8 * r, k = q + 8 * s2;
+          f[r][s2] = (i < sb && k <= i) ? D[k * kPanelLd + i] : T(0);
+          x[r][s2] = i == k ? T(1) : T(0);
+        }
+      const double thr = lane < sb ? 1e-13 * fabs(d0[c + lane]) : 0.0;  // drop threshold of column `lane`
 #pragma unroll
       for (int j = 0; j < kSub; ++j) {
         if (j < sb) {
-          const double dj = static_cast<double>(read_lane(reg[j], j));
+          const int jp = j & 7, js = j >> 3;
+          const double dj = static_cast<double>(read_lane(f[js][js], 9 * jp));
           const bool finite = isfinite(dj);
           const bool drop = finite && dj <= read_lane(thr, j);
           const bool bad = !drop && (!finite || dj < 1e-30);
@@ -148,28 +157,42 @@ __global__ __launch_bounds__(kPanelThreads) void potrf_panel_kernel(T* __restric
             if (bad) atomicOr(&flags->status, 1);
             xdiag[c + j] = rpiv;
           }
-          // factor lanes below j: l = a * rpiv; lane j: the pivot; inverse
-          // lanes: x_j *= rpiv. Then reg[k] -= reg[j] L(k, j) for k > j,
-          // L(k, j) broadcast from lane k.
-          T v = reg[j] * rpiv;
-          if (fac && me == j) v = piv;
-          if (!fac || me >= j) reg[j] = v;
-          T col[kSub];
+          if (q == jp) {  // column j of L: the pivot, and l = a / pivot below it
 #pragma unroll
-          for (int k = j + 1; k < kSub; ++k) col[k] = bcast_lane(reg[j], k);
+            for (int r = 0; r < 4; ++r) {
+              const int i = p + 8 * r;
+              f[r][js] = i == j ? piv : (i > j ? f[r][js] * rpiv : f[r][js]);
+            }
+          }
+          if (p == jp) {  // row j of X
 #pragma unroll
-          for (int k = j + 1; k < kSub; ++k) reg[k] -= reg[j] * col[k];
+            for (int s2 = 0; s2 < 4; ++s2) x[js][s2] *= rpiv;
+          }
+          T lr[4], lc[4], xr[4];
+#pragma unroll
+          for (int r = 0; r < 4; ++r) lr[r] = bcast_from(f[r][js], p + 8 * jp);   // L(p + 8r, j)
+#pragma unroll
+          for (int s2 = 0; s2 < 4; ++s2) {
+            lc[s2] = bcast_from(f[s2][js], q + 8 * jp);  // L(q + 8 s2, j)
+            xr[s2] = bcast_from(x[js][s2], jp + 8 * q);  // X(j, q + 8 s2)
+          }
+#pragma unroll
+          for (int r = 0; r < 4; ++r)
+#pragma unroll
+            for (int s2 = 0; s2 < 4; ++s2) {
+              if (q + 8 * s2 > j) f[r][s2] -= lr[r] * lc[s2];
+              if (p + 8 * r > j) x[r][s2] -= lr[r] * xr[s2];
+            }
         }
       }
-      if (fac) {  // row me of L
 #pragma unroll
-        for (int k = 0; k < kSub; ++k)
-          if (k <= me && me < sb) D[k * kPanelLd + me] = reg[k];
-      } else {  // X(i, me) for i > me, into the upper triangle
+      for (int r = 0; r < 4; ++r)
 #pragma unroll
-        for (int i = 1; i < kSub; ++i)
-          if (i > me && i < sb) D[i * kPanelLd + me] = reg[i];
-      }
+        for (int s2 = 0; s2 < 4; ++s2) {
+          const int i = p + 8 * r, k = q + 8 * s2;
+          if (i < sb && k <= i) D[k * kPanelLd + i] = f[r][s2];  // L(i, k)
+          if (i < sb && k < i) D[i * kPanelLd + k] = x[r][s2];   // X(i, k), above the diagonal
+        }
     }
     __syncthreads();
 
@@ -321,10 +344,11 @@ bool configure_lds() {
   return ok;
 }
 
+}  // namespace
+
 // Event pairs recorded around kernels while timing is on. Recording does
 // not wait; the elapsed times are collected (after one stream sync) by
 // sov_kernel_times, so timing can stay on inside a timed region.
-enum TimedKind { kTimeSyrk = 0, kTimePanel = 1, kTimeSolve = 2 };
 struct TimedPair {
   hipEvent_t a = nullptr, b = nullptr;
   int kind = 0;
@@ -366,7 +390,6 @@ void timed_end(TimedPair* p, hipStream_t s, int launches, double flops) {
   (void)hipEventRecord(p->b, s);
 }
 
-}  // namespace
 
 void set_kernel_timing(bool on) { g_timing.store(on); }
 bool kernel_timing() { return g_timing.load(); }
@@ -392,6 +415,16 @@ void collect_kernel_times(hipStream_t s) {
     } else if (e.kind == kTimePanel) {
       t.panel_ms += ms;
       t.panel_launches += e.launches;
+    } else if (e.kind == kTimeStrip) {
+      t.strip_ms += ms;
+    } else if (e.kind == kTimeInverse) {
+      t.inverse_ms += ms;
+    } else if (e.kind == kTimeAssemble) {
+      t.assemble_ms += ms;
+    } else if (e.kind == kTimeSyrk32) {
+      t.syrk32_ms += ms;
+      t.syrk32_launches += e.launches;
+      t.syrk32_flops += e.flops;
     } else {
       t.solve_ms += ms;
     }
@@ -480,16 +513,16 @@ int DenseCholesky<T>::factor(T* a, int m, const double* diag0, hipStream_t s) {
     const int strip = std::min(kPanel, below);
     const int rest = below - strip;
     SOV_HIP_TRY(hipStreamWaitEvent(s, ev_bulk_, 0));  // the strip was part of the previous bulk update
-    TimedPair* ts = timed_begin(s, kTimeSyrk);
+    TimedPair* ts = timed_begin(s, kTimeStrip);
     T* a_strip = a + (k0 + nb) + static_cast<long long>(k0 + nb) * lda;
     SOV_HIP_TRY((gemm_update<T, false, false>(below, strip, nb, L21, lda, L21, lda, a_strip, lda, T(-1), s)));
-    timed_end(ts, s, 1, static_cast<double>(below) * strip * nb * 2.0 - static_cast<double>(strip) * (strip - 1) * nb);
+    timed_end(ts, s, 1, 0.0);
     if (rest > 0) {
       SOV_HIP_TRY(hipEventRecord(ev_panel_, s));
       SOV_HIP_TRY(hipStreamWaitEvent(side_, ev_panel_, 0));
       T* a22 = a + (k0 + nb + strip) + static_cast<long long>(k0 + nb + strip) * lda;
       const T* l21r = L21 + strip;
-      TimedPair* tb = timed_begin(side_, kTimeSyrk);
+      TimedPair* tb = timed_begin(side_, sizeof(T) == 8 ? kTimeSyrk : kTimeSyrk32);
       SOV_HIP_TRY((gemm_update<T, false, true>(rest, rest, nb, l21r, lda, l21r, lda, a22, lda, T(-1), side_)));
       timed_end(tb, side_, 1, static_cast<double>(rest) * (rest + 1) * nb);
       SOV_HIP_TRY(hipEventRecord(ev_bulk_, side_));
@@ -509,7 +542,7 @@ template <typename T>
 int DenseCholesky<T>::build_super_inverses(const T* a, int m, hipStream_t s) {
   const long long lda = m, ks = kSuper, sq = ks * ks;
   const int nfull = m / kSuper, rem = m - nfull * kSuper;
-  TimedPair* tp = timed_begin(s, kTimePanel);
+  TimedPair* tp = timed_begin(s, kTimeInverse);
   int launches = 0;
   for (long long b = kPanel; b < kSuper; b *= 2) {
     const int npairs = static_cast<int>(kSuper / (2 * b));
@@ -593,9 +626,10 @@ int sov_kernel_times(double* out) {
   if (!out) return SOV_ERR_ARG;
   if (sov::device_ready()) sov::collect_kernel_times(sov::stream());
   sov::KernelTimes& t = sov::kernel_times();
-  const double v[7] = {t.syrk_ms, t.syrk_launches, t.syrk_flops, t.panel_ms, t.solve_ms, t.assemble_ms,
-                       t.panel_launches};
-  for (int i = 0; i < 7; ++i) out[i] = v[i];
+  const double v[12] = {t.syrk_ms,        t.syrk_launches, t.syrk_flops, t.panel_ms,    t.solve_ms,
+                        t.assemble_ms,    t.panel_launches, t.strip_ms, t.inverse_ms, t.syrk32_ms,
+                        t.syrk32_launches, t.syrk32_flops};
+  for (int i = 0; i < 12; ++i) out[i] = v[i];
   t = sov::KernelTimes{};
   return SOV_OK;
 }

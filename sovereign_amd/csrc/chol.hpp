@@ -59,10 +59,26 @@ class DenseCholesky {
 struct KernelTimes {
   double syrk_ms = 0.0, syrk_launches = 0.0, syrk_flops = 0.0;
   double panel_ms = 0.0, solve_ms = 0.0, assemble_ms = 0.0, panel_launches = 0.0;
+  double strip_ms = 0.0, inverse_ms = 0.0;
+  double syrk32_ms = 0.0, syrk32_launches = 0.0, syrk32_flops = 0.0;  // FP32 factorizations' SYRK
 };
 void set_kernel_timing(bool on);
 bool kernel_timing();
 KernelTimes& kernel_times();
 void collect_kernel_times(hipStream_t s);
+
+// Event pair around work on a stream (nullptr when timing is off).
+enum TimedKind {
+  kTimeSyrk = 0,  // FP64 bulk trailing update
+  kTimePanel = 1,
+  kTimeSolve = 2,
+  kTimeStrip = 3,
+  kTimeInverse = 4,
+  kTimeAssemble = 5,
+  kTimeSyrk32 = 6  // FP32 bulk trailing update
+};
+struct TimedPair;
+TimedPair* timed_begin(hipStream_t s, int kind);
+void timed_end(TimedPair* p, hipStream_t s, int launches, double flops);
 
 }  // namespace sov

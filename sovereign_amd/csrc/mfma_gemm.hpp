@@ -149,26 +149,31 @@ __global__ __launch_bounds__(256) void gemm_update_kernel(int M, int N, int K, c
       const int krow = (kk + lk) * kLdsStride;
       const T a0 = As[krow + wr + lrow], a1 = As[krow + wr + 16 + lrow];
       const T b0 = Bs[krow + wc + lrow], b1 = Bs[krow + wc + 16 + lrow];
-      acc[0][0] = Mfma<T>::step(a0, b0, acc[0][0]);
-      acc[0][1] = Mfma<T>::step(a0, b1, acc[0][1]);
-      acc[1][0] = Mfma<T>::step(a1, b0, acc[1][0]);
-      acc[1][1] = Mfma<T>::step(a1, b1, acc[1][1]);
+      // The B tile goes in as the instruction's first operand, so each
+      // 16 x 16 result comes out transposed: a lane holds one row of C.
+      acc[0][0] = Mfma<T>::step(b0, a0, acc[0][0]);
+      acc[0][1] = Mfma<T>::step(b1, a0, acc[0][1]);
+      acc[1][0] = Mfma<T>::step(b0, a1, acc[1][0]);
+      acc[1][1] = Mfma<T>::step(b1, a1, acc[1][1]);
     }
   }
 
-  // Lane holds column (lane & 15) of each block; the rows of its registers
-  // depend on the instruction (Mfma<T>::row). Operands: A(i, k) and B(k, j)
-  // sit on lane i + 16 k and j + 16 k.
+  // Operands X(i, k) and Y(k, j) of an instruction sit on lanes i + 16 k and
+  // j + 16 k; result (i, j) on lane j + 16 (...), register g with i =
+  // Mfma<T>::row(lane >> 4, g). Here X = B-tile rows (C columns) and Y =
+  // A-tile rows (C rows), so lane l holds C row (l & 15) of the block for
+  // four C columns: one store instruction writes 16 consecutive rows of each
+  // of four columns.
 #pragma unroll
   for (int i = 0; i < 2; ++i) {
+    const int gr = r0 + wr + 16 * i + lrow;
+    if (gr >= M) continue;
 #pragma unroll
     for (int j = 0; j < 2; ++j) {
-      const int gc = c0 + wc + 16 * j + lrow;
-      if (gc >= N) continue;
 #pragma unroll
       for (int g = 0; g < 4; ++g) {
-        const int gr = r0 + wr + 16 * i + Mfma<T>::row(lk, g);
-        if (gr >= M || (LOWER && gr < gc)) continue;
+        const int gc = c0 + wc + 16 * j + Mfma<T>::row(lk, g);
+        if (gc >= N || (LOWER && gr < gc)) continue;
         if (STORE) C[gr + static_cast<long long>(gc) * ldc] = alpha * acc[i][j][g];
         else C[gr + static_cast<long long>(gc) * ldc] += alpha * acc[i][j][g];
       }

@@ -318,6 +318,7 @@ int NormalSolver::init(sov_mat* a) {
 template <typename T>
 int NormalSolver::assemble(const double* d_dev, T* M, hipStream_t s) {
   if (m_ == 0) return SOV_OK;
+  TimedPair* tp = timed_begin(s, kTimeAssemble);
   fill_lower_kernel<T><<<m_, 256, 0, s>>>(M, m_);
   SOV_LAUNCH_CHECK();
   if (dense_a_) {
@@ -333,6 +334,7 @@ int NormalSolver::assemble(const double* d_dev, T* M, hipStream_t s) {
                                                                       A_->c_val, d_dev, M, diag0_, kNormalReg);
     SOV_LAUNCH_CHECK();
   }
+  timed_end(tp, s, dense_a_ ? 4 : 2, 0.0);
   return SOV_OK;
 }
 
