@@ -23,6 +23,8 @@
 #include "internal.hpp"
 
 #include <algorithm>
+#include <cstdio>
+#include <cstdlib>
 #include <cmath>
 #include <vector>
 
@@ -34,6 +36,7 @@ constexpr int kThreads = 1024;
 constexpr int kWaves = kThreads / 64;
 constexpr int kMaxRows = 272;
 constexpr int kStage = 1024;  // doubles of LDS staging (vectors, partial sums)
+constexpr int kTraceRows = 200, kTraceCols = 16;  // SOVEREIGN_BATCH_TRACE record
 
 struct BatchMatrix {
   int m, n;
@@ -83,7 +86,7 @@ struct Lp {
   const double *b, *c, *u;
   // workspace vectors (global, this LP only)
   double *x, *w, *z, *v, *rc, *ru, *D, *dx, *dw, *dz, *dv, *dxa, *dwa, *dza, *dva, *t;
-  double *y, *rb, *dy, *r, *p, *q, *zz, *rhs, *diag0;
+  double *y, *rb, *dy, *r, *p, *q, *zz, *rhs, *sc;  // sc: Jacobi scale of A D A^T
 };
 
 // out_j = (A^T vec)_j for j < n; vec (m) is staged in LDS first.
@@ -131,10 +134,32 @@ __device__ void gemv_n(const Lp& L, const double* vec, const double* base, doubl
   __syncthreads();
 }
 
-// M (packed lower, LDS, FP32) = A diag(D) A^T on MFMA; diag0 = its diagonal.
+// Jacobi-scaled normal matrix in LDS (packed lower, FP32):
+//   Mt = S A D A^T S,  S = diag(sc),  sc_i = (A D A^T)_ii^(-1/2),
+// assembled on MFMA from the operands A(i, k) sqrt(D_k) sc_i, so Mt has a
+// unit diagonal whatever the spread of D; the FP32 factorization of Mt is
+// then accurate relative to its own scale. L.t holds sqrt(D) meanwhile.
 __device__ void assemble(const Lp& L, float* M, double reg) {
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   const int lrow = lane & 15, lk = lane >> 4;
+  for (int j = tid; j < L.n; j += kThreads) L.t[j] = sqrt(fmax(L.D[j], 0.0));
+  __syncthreads();
+  {  // diagonal in FP64: four threads per row
+    const int row = tid >> 2, part = tid & 3;
+    for (int i0 = 0; i0 < L.m; i0 += kThreads / 4) {
+      const int i = i0 + row;
+      double d = 0.0;
+      if (i < L.m)
+        for (int k = part; k < L.n; k += 4) {
+          const double a = L.A[i + static_cast<long long>(k) * L.m];
+          d = fma(a * a, L.D[k], d);
+        }
+      d += __shfl_xor(d, 1);
+      d += __shfl_xor(d, 2);
+      if (part == 0 && i < L.m) L.sc[i] = d + reg > 0.0 ? 1.0 / sqrt(d + reg) : 1.0;
+    }
+  }
+  __syncthreads();
   const int T = (L.m + 15) / 16;
   const int tiles = T * (T + 1) / 2;
   for (int t = wv; t < tiles; t += kWaves) {
@@ -144,6 +169,7 @@ __device__ void assemble(const Lp& L, float* M, double reg) {
     const int tj = t - ti * (ti + 1) / 2;
     const int r0 = ti * 16, c0 = tj * 16;
     const int ra = r0 + lrow, cb = c0 + lrow;
+    const double sa = ra < L.m ? L.sc[ra] : 0.0, sbv = cb < L.m ? L.sc[cb] : 0.0;
     flt4_t acc = {0.f, 0.f, 0.f, 0.f};
     for (int k0 = 0; k0 < L.n; k0 += 16) {
       float av[4], bv[4];
@@ -151,9 +177,9 @@ __device__ void assemble(const Lp& L, float* M, double reg) {
       for (int u = 0; u < 4; ++u) {
         const int k = k0 + 4 * u + lk;
         const bool kin = k < L.n;
-        const float dk = kin ? static_cast<float>(L.D[k]) : 0.f;
-        av[u] = (kin && ra < L.m) ? L.Af[ra + static_cast<long long>(k) * L.m] * dk : 0.f;
-        bv[u] = (kin && cb < L.m) ? L.Af[cb + static_cast<long long>(k) * L.m] : 0.f;
+        const double sd = kin ? L.t[k] : 0.0;
+        av[u] = (kin && ra < L.m) ? L.Af[ra + static_cast<long long>(k) * L.m] * static_cast<float>(sd * sa) : 0.f;
+        bv[u] = (kin && cb < L.m) ? L.Af[cb + static_cast<long long>(k) * L.m] * static_cast<float>(sd * sbv) : 0.f;
       }
 #pragma unroll
       for (int u = 0; u < 4; ++u) acc = __builtin_amdgcn_mfma_f32_16x16x4f32(av[u], bv[u], acc, 0, 0, 0);
@@ -162,16 +188,16 @@ __device__ void assemble(const Lp& L, float* M, double reg) {
 #pragma unroll
     for (int g = 0; g < 4; ++g) {
       const int i = r0 + 4 * lk + g, j = c0 + lrow;
-      if (i < L.m && j < L.m && i >= j) M[packed_index(i, j, L.m)] = acc[g] + (i == j ? static_cast<float>(reg) : 0.f);
+      if (i < L.m && j < L.m && i >= j)
+        M[packed_index(i, j, L.m)] = acc[g] + (i == j ? static_cast<float>(reg * L.sc[i] * L.sc[i]) : 0.f);
     }
   }
   __syncthreads();
-  for (int i = tid; i < L.m; i += kThreads) L.diag0[i] = static_cast<double>(M[packed_index(i, i, L.m)]);
-  __syncthreads();
 }
 
-// In-place FP32 Cholesky of the packed lower triangle in LDS. Returns the
-// number of dropped pivots.
+// In-place FP32 Cholesky of the packed lower triangle in LDS (unit diagonal
+// before elimination, so the reference's drop rule reads pivot <= 1e-13).
+// Returns the number of dropped pivots.
 __device__ int cholesky(const Lp& L, float* M) {
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   const int m = L.m;
@@ -179,7 +205,7 @@ __device__ int cholesky(const Lp& L, float* M) {
   for (int k = 0; k < m; ++k) {
     const long long kk = packed_index(k, k, m);
     const double dk = static_cast<double>(M[kk]);
-    const bool drop = !(dk > 1e-13 * fabs(L.diag0[k])) && isfinite(dk);
+    const bool drop = !(dk > 1e-13) && isfinite(dk);
     const float piv = drop ? 1e18f : (isfinite(dk) ? static_cast<float>(sqrt(dk)) : 1.f);
     dropped += drop ? 1 : 0;
     for (int i = k + 1 + tid; i < m; i += kThreads) M[kk + (i - k)] /= piv;
@@ -195,25 +221,74 @@ __device__ int cholesky(const Lp& L, float* M) {
   return dropped;
 }
 
-// x <- L^-T L^-1 x in LDS (vec, FP64), with the FP32 factor.
+// In place: the packed factor L becomes X = L^-1 (lower). Column j, from
+// the last: X(j, j) = 1 / L(j, j), X(i, j) = -X(j, j) sum_{k=j+1..i} X(i, k)
+// L(k, j), with the columns right of j already inverted. Four threads per
+// row share the sum; tmp (LDS, m doubles) holds the column until all reads
+// of L(., j) are done.
+__device__ void invert_lower(const Lp& L, float* M, double* tmp) {
+  const int tid = threadIdx.x, m = L.m;
+  const int row = tid >> 2, part = tid & 3;
+  for (int j = m - 1; j >= 0; --j) {
+    const long long jj = packed_index(j, j, m);
+    const float xjj = 1.f / M[jj];  // read before anyone rewrites it below
+    for (int i0 = j + 1; i0 < m; i0 += kThreads / 4) {
+      const int i = i0 + row;
+      float acc = 0.f;
+      if (i < m) {
+        // X(i, k) at packed_index(i, k): the stride to k + 1 is m - k - 1
+        long long idx = packed_index(i, j + 1 + part, m);
+        for (int k = j + 1 + part; k <= i; k += 4) {
+          acc += M[idx] * M[jj + (k - j)];
+          idx += (m - k - 1) + (m - k - 2) + (m - k - 3) + (m - k - 4);
+        }
+      }
+      acc += __shfl_xor(acc, 1);
+      acc += __shfl_xor(acc, 2);
+      if (part == 0 && i < m) tmp[i] = acc;
+    }
+    __syncthreads();
+    for (int i = j + 1 + tid; i < m; i += kThreads) M[jj + (i - j)] = -xjj * static_cast<float>(tmp[i]);
+    if (tid == 0) M[jj] = xjj;
+    __syncthreads();
+  }
+}
+
+// vec <- M^-1 vec = S X^T X S vec (vec in LDS, FP64), with X = inverse of
+// the FP32 factor of the scaled matrix; tmp (LDS, m doubles) is scratch.
 __device__ void precondition(const Lp& L, const float* M, double* vec, double* tmp) {
-  const int tid = threadIdx.x;
-  const int m = L.m;
+  const int tid = threadIdx.x, m = L.m;
+  const int row = tid >> 2, part = tid & 3;
   __syncthreads();
-  for (int k = 0; k < m; ++k) {  // forward: tmp = L^-1 vec
-    const long long kk = packed_index(k, k, m);
-    const double yk = vec[k] / static_cast<double>(M[kk]);
-    if (tid == 0) tmp[k] = yk;
-    for (int i = k + 1 + tid; i < m; i += kThreads) vec[i] -= static_cast<double>(M[kk + (i - k)]) * yk;
-    __syncthreads();
+  for (int i = tid; i < m; i += kThreads) vec[i] *= L.sc[i];
+  __syncthreads();
+  for (int i0 = 0; i0 < m; i0 += kThreads / 4) {  // tmp = X vec: row i of X
+    const int i = i0 + row;
+    double acc = 0.0;
+    if (i < m) {
+      long long idx = packed_index(i, part, m);
+      for (int k = part; k <= i; k += 4) {
+        acc = fma(static_cast<double>(M[idx]), vec[k], acc);
+        idx += (m - k - 1) + (m - k - 2) + (m - k - 3) + (m - k - 4);
+      }
+    }
+    acc += __shfl_xor(acc, 1);
+    acc += __shfl_xor(acc, 2);
+    if (part == 0 && i < m) tmp[i] = acc;
   }
-  for (int k = m - 1; k >= 0; --k) {  // backward: vec = L^-T tmp
-    const long long kk = packed_index(k, k, m);
-    const double xk = tmp[k] / static_cast<double>(M[kk]);
-    if (tid == 0) vec[k] = xk;
-    for (int j = tid; j < k; j += kThreads) tmp[j] -= static_cast<double>(M[packed_index(k, j, m)]) * xk;
-    __syncthreads();
+  __syncthreads();
+  for (int k0 = 0; k0 < m; k0 += kThreads / 4) {  // vec = X^T tmp: column k of X
+    const int k = k0 + row;
+    double acc = 0.0;
+    if (k < m) {
+      const long long base = packed_index(k, k, m);
+      for (int i = k + part; i < m; i += 4) acc = fma(static_cast<double>(M[base + (i - k)]), tmp[i], acc);
+    }
+    acc += __shfl_xor(acc, 1);
+    acc += __shfl_xor(acc, 2);
+    if (part == 0 && k < m) vec[k] = acc * L.sc[k];
   }
+  __syncthreads();
 }
 
 // q = A D A^T p (FP64, matrix-free); p, q global (m).
@@ -350,7 +425,8 @@ __global__ __launch_bounds__(kThreads) void batch_ipm_kernel(const BatchMatrix* 
                                                              unsigned* queue, const double* __restrict__ b_all,
                                                              const double* __restrict__ c_all,
                                                              const double* __restrict__ u_all, double* ws,
-                                                             double* x_all, sov_batch_result_dev* out, BatchOpts o) {
+                                                             double* x_all, sov_batch_result_dev* out, BatchOpts o,
+                                                             double* trace, int trace_lp) {
   extern __shared__ __align__(16) unsigned char smem[];
   double* stage = reinterpret_cast<double*>(smem);           // kStage doubles
   double* red = stage + kStage;                              // 64 doubles
@@ -380,7 +456,7 @@ __global__ __launch_bounds__(kThreads) void batch_ipm_kernel(const BatchMatrix* 
       *pp = wsp;
       wsp += L.n;
     }
-    double** mv[] = {&L.y, &L.rb, &L.dy, &L.r, &L.p, &L.q, &L.zz, &L.rhs, &L.diag0};
+    double** mv[] = {&L.y, &L.rb, &L.dy, &L.r, &L.p, &L.q, &L.zz, &L.rhs, &L.sc};
     for (double** pp : mv) {
       *pp = wsp;
       wsp += L.m;
@@ -416,8 +492,18 @@ __global__ __launch_bounds__(kThreads) void batch_ipm_kernel(const BatchMatrix* 
 
     int status = 5, iters = 0, pcg_total = 0;
     double pobj = 0.0, gap = 0.0, pres = 0.0, dres = 0.0;
+    // optional per-iteration record of one LP (kTraceCols doubles a row)
+    double* tr = (trace && lp == trace_lp) ? trace : nullptr;
+    auto note = [&](int it, int col, double v) {
+      if (tr && tid == 0 && it < kTraceRows) tr[it * kTraceCols + col] = v;
+    };
+    int stalled = 0;  // consecutive iterations whose two PCG solves both hit the cap
+    int last_pcg = 0;
+    double best = 1e300;
+    int best_it = 0;
     for (int it = 0; it < o.max_iter; ++it) {
       __syncthreads();
+      long long t0 = clock64();
       // residuals
       gemv_n(L, L.x, nullptr, L.rb, stage);  // rb = A x (then b - A x)
       gemv_t(L, L.y, L.t, stage);            // t = A^T y
@@ -462,6 +548,10 @@ __global__ __launch_bounds__(kThreads) void batch_ipm_kernel(const BatchMatrix* 
       dres = rcmax / (1.0 + cnorm);
       gap = fabs(pobj - dobj) / (1.0 + fabs(pobj));
       iters = it;
+      note(it, 0, it);
+      note(it, 1, pres);
+      note(it, 2, dres);
+      note(it, 3, gap);
       if (pres < o.tol && dres < o.tol && gap < o.tol) {
         status = 0;
         break;
@@ -470,7 +560,20 @@ __global__ __launch_bounds__(kThreads) void batch_ipm_kernel(const BatchMatrix* 
         status = 6;
         break;
       }
+      // Stop early when the FP32-preconditioned solves have stalled (the
+      // caller finishes such LPs on a path with FP64 factorizations): the
+      // solves keep hitting their cap, or 30 iterations without progress.
+      const double metric = fmax(pres, fmax(dres, gap));
+      if (metric < best) {
+        best = metric;
+        best_it = it;
+      }
+      if (stalled >= 3 || it - best_it > 30) {
+        status = 6;
+        break;
+      }
       const double mu = xz / fmax(nact + nb, 1.0);
+      note(it, 4, mu);
       // scaling D = (z/x + v/w)^-1
       for (int j = tid; j < n; j += kThreads) {
         double dj = 0.0;
@@ -482,8 +585,17 @@ __global__ __launch_bounds__(kThreads) void batch_ipm_kernel(const BatchMatrix* 
         L.D[j] = dj;
       }
       __syncthreads();
+      long long t1 = clock64();
+      note(it, 9, static_cast<double>(t1 - t0));
       assemble(L, M, 0.0);
-      cholesky(L, M);
+      t0 = clock64();
+      note(it, 10, static_cast<double>(t0 - t1));
+      note(it, 15, cholesky(L, M));
+      t1 = clock64();
+      note(it, 11, static_cast<double>(t1 - t0));
+      invert_lower(L, M, stage);
+      t0 = clock64();
+      note(it, 12, static_cast<double>(t0 - t1));
       // predictor: r1 = -x z, r2 = -w v (in dxa/dwa scratch: use dza/dva as rhs storage)
       for (int j = tid; j < n; j += kThreads) {
         L.dza[j] = -L.x[j] * L.z[j];
@@ -496,7 +608,14 @@ __global__ __launch_bounds__(kThreads) void batch_ipm_kernel(const BatchMatrix* 
         L.dwa[j] = L.dva[j];
       }
       __syncthreads();
-      pcg_total += newton(L, M, stage, red, L.dxa, L.dwa, L.dx, L.dw, L.dz, L.dv, o);
+      {
+        const int k1 = newton(L, M, stage, red, L.dxa, L.dwa, L.dx, L.dw, L.dz, L.dv, o);
+        pcg_total += k1;
+        last_pcg = k1;
+        note(it, 7, k1);
+        t1 = clock64();
+        note(it, 13, static_cast<double>(t1 - t0));
+      }
       double ap, ad;
       max_steps(L, L.dx, L.dw, L.dz, L.dv, red, &ap, &ad);
       double mua = 0.0;
@@ -513,10 +632,19 @@ __global__ __launch_bounds__(kThreads) void batch_ipm_kernel(const BatchMatrix* 
         L.dwa[j] = bounded(L, j) ? sigma * mu - L.w[j] * L.v[j] - L.dw[j] * L.dv[j] : 0.0;
       }
       __syncthreads();
-      pcg_total += newton(L, M, stage, red, L.dxa, L.dwa, L.dx, L.dw, L.dz, L.dv, o);
+      {
+        const int k2 = newton(L, M, stage, red, L.dxa, L.dwa, L.dx, L.dw, L.dz, L.dv, o);
+        pcg_total += k2;
+        note(it, 8, k2);
+        stalled = (k2 >= o.pcg_max && last_pcg >= o.pcg_max) ? stalled + 1 : 0;
+        t0 = clock64();
+        note(it, 14, static_cast<double>(t0 - t1));
+      }
       max_steps(L, L.dx, L.dw, L.dz, L.dv, red, &ap, &ad);
       ap = fmin(1.0, o.tau * ap);
       ad = fmin(1.0, o.tau * ad);
+      note(it, 5, ap);
+      note(it, 6, ad);
       for (int j = tid; j < n; j += kThreads) {
         if (!active(L, j)) continue;
         L.x[j] = fmax(L.x[j] + ap * L.dx[j], 1e-14);
@@ -681,8 +809,17 @@ int sov_batch_solve(sov_batch* h, int32_t count, const int32_t* matrix_id, const
   SOV_HIP_TRY(hipEventCreate(&e0));
   SOV_HIP_TRY(hipEventCreate(&e1));
   (void)hipEventRecord(e0, s);
+  // SOVEREIGN_BATCH_TRACE=k records the iterations of LP k (stderr).
+  const char* tenv = std::getenv("SOVEREIGN_BATCH_TRACE");
+  const int trace_lp = tenv ? std::atoi(tenv) : -1;
+  double* d_trace = nullptr;
+  if (trace_lp >= 0 && trace_lp < count) {
+    d_trace = tmp.get<double>(kTraceRows * kTraceCols);
+    if (!d_trace) return SOV_ERR_NOMEM;
+    SOV_HIP_TRY(hipMemsetAsync(d_trace, 0, sizeof(double) * kTraceRows * kTraceCols, s));
+  }
   batch_ipm_kernel<<<grid, kThreads, lds_bytes(mmax), s>>>(d_mats, d_desc, count, d_queue, d_b, d_c, d_u, d_ws, d_x,
-                                                          d_out, o);
+                                                          d_out, o, d_trace, trace_lp);
   const hipError_t le = hipGetLastError();
   (void)hipEventRecord(e1, s);
   if (le != hipSuccess) {
@@ -696,6 +833,19 @@ int sov_batch_solve(sov_batch* h, int32_t count, const int32_t* matrix_id, const
   if (x_all) SOV_HIP_TRY(hipMemcpyAsync(x_all, d_x, x_total * sizeof(double), hipMemcpyDeviceToHost, s));
   SOV_HIP_TRY(hipStreamSynchronize(s));
   (void)hipEventElapsedTime(&h->last_ms, e0, e1);
+  if (d_trace) {
+    std::vector<double> t(kTraceRows * kTraceCols);
+    SOV_HIP_TRY(hipMemcpy(t.data(), d_trace, sizeof(double) * t.size(), hipMemcpyDeviceToHost));
+    for (int r = 0; r < kTraceRows; ++r) {
+      const double* q = t.data() + r * kTraceCols;
+      if (r > 0 && q[0] == 0.0) break;
+      std::fprintf(stderr,
+                   "[batch %d] it=%d pres=%.2e dres=%.2e gap=%.2e mu=%.2e ap=%.3f ad=%.3f pcg=%d/%d drop=%d "
+                   "cycles res=%.0f asm=%.0f chol=%.0f inv=%.0f newton=%.0f/%.0f\n",
+                   trace_lp, static_cast<int>(q[0]), q[1], q[2], q[3], q[4], q[5], q[6], static_cast<int>(q[7]),
+                   static_cast<int>(q[8]), static_cast<int>(q[15]), q[9], q[10], q[11], q[12], q[13], q[14]);
+    }
+  }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   for (int k = 0; k < count; ++k) {

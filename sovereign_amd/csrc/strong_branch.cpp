@@ -197,6 +197,26 @@ int solve_children_one_by_one(const sov_lp_model& mdl, const std::vector<ChildSp
   return SOV_OK;
 }
 
+// Children the batched kernel could not finish (status 5 or 6: it stops early
+// when its FP32-preconditioned solves stall) go through the per-LP path,
+// whose late iterations factor in FP64.
+int finish_unconverged(const sov_lp_model& mdl, const std::vector<ChildSpec>& kids, const DenseForm& F,
+                       const sov_ipm_options& opt, std::vector<ChildOut>& out, int* lp_solves) {
+  std::vector<ChildSpec> redo;
+  std::vector<int> where;
+  for (std::size_t k = 0; k < out.size(); ++k)
+    if (out[k].status == 5 || out[k].status == 6) {
+      redo.push_back(kids[k]);
+      where.push_back(static_cast<int>(k));
+    }
+  if (redo.empty()) return SOV_OK;
+  std::vector<ChildOut> again;
+  const int rc = solve_children_one_by_one(mdl, redo, F, opt, again, lp_solves);
+  if (rc != SOV_OK) return rc;
+  for (std::size_t q = 0; q < where.size(); ++q) out[where[q]] = std::move(again[q]);
+  return SOV_OK;
+}
+
 double frac_part(double v) { return std::min(v - std::floor(v), std::ceil(v) - v); }
 
 // fractional_vars + the closeness-to-1/2 order of branch_and_bound.cpp:731-741.
@@ -275,6 +295,7 @@ int sov_strong_branch(const sov_lp_model* node, const double* node_lb, const dou
     rc = sov_batch_add_matrix(batch, F.m, F.n, F.A.data(), &mid);
     if (rc == SOV_OK) rc = solve_children(batch, mid, F, kids, *opt, res, kernel_ms, nullptr);
     sov_batch_free(batch);
+    if (rc == SOV_OK) rc = finish_unconverged(*node, kids, F, *opt, res, nullptr);
   } else {
     rc = solve_children_one_by_one(*node, kids, F, *opt, res, nullptr);
   }
@@ -330,8 +351,10 @@ int sov_bnb_solve(const sov_lp_model* milp, const int8_t* is_int, const sov_bnb_
   int lp_solves = 0;
   double sb_seconds = 0.0;
   auto solve_set = [&](const std::vector<ChildSpec>& kids, std::vector<ChildOut>& outv) {
-    if (bo->child_solver == SOV_CHILD_BATCHED)
-      return solve_children(batch.get(), mat_id, F, kids, *opt, outv, &kernel_ms, &lp_solves);
+    if (bo->child_solver == SOV_CHILD_BATCHED) {
+      const int rc = solve_children(batch.get(), mat_id, F, kids, *opt, outv, &kernel_ms, &lp_solves);
+      return rc != SOV_OK ? rc : finish_unconverged(*milp, kids, F, *opt, outv, &lp_solves);
+    }
     return solve_children_one_by_one(*milp, kids, F, *opt, outv, &lp_solves);
   };
   // Minimisation view: bounds and incumbent in F.sense * objective.
