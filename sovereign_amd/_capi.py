@@ -136,6 +136,12 @@ class LpModelC(C.Structure):
     ]
 
 
+class MlpShape(C.Structure):
+    _fields_ = [("in_", C.c_int32), ("hidden1", C.c_int32), ("hidden2", C.c_int32), ("pad", C.c_int32)]
+
+
+_f32p = C.POINTER(C.c_float)
+
 SIGNATURES = {
     "sov_dev_probe": (C.c_int, [C.POINTER(DevInfo)]),
     "sov_gpu_operations": (C.c_uint64, []),
@@ -179,6 +185,15 @@ SIGNATURES = {
     "sov_bnb_default_options": (None, [C.POINTER(BnbOptions)]),
     "sov_bnb_solve": (C.c_int, [C.POINTER(LpModelC), C.POINTER(C.c_int8), C.POINTER(BnbOptions),
                                 C.POINTER(IpmOptions), C.POINTER(BnbResult), _f64p]),
+    "sov_mlp_create": (C.c_int, [C.POINTER(MlpShape), C.c_uint64, C.POINTER(_vp)]),
+    "sov_mlp_param_count": (C.c_int64, [_vp]),
+    "sov_mlp_get_params": (C.c_int, [_vp, _f32p]),
+    "sov_mlp_set_params": (C.c_int, [_vp, _f32p]),
+    "sov_mlp_forward": (C.c_int, [_vp, _f32p, C.c_int32, _f32p]),
+    "sov_mlp_gradients": (C.c_int, [_vp, _f32p, _f32p, C.c_int32, _f32p, _f32p]),
+    "sov_mlp_train": (C.c_int, [_vp, _f32p, _f32p, C.c_int64, C.c_int32, C.c_int32, C.c_float, C.c_uint64,
+                                _f32p]),
+    "sov_mlp_free": (C.c_int, [_vp]),
 }
 
 _lib = None
