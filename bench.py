@@ -29,6 +29,9 @@ with the exponent fitted between them.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
                        [--paths mixed,fp64] [--no-branching]
+       python bench.py --copy-probe   (a plain 256 MiB device copy, three
+                                       times: the calibration run for the
+                                       FETCH_SIZE / WRITE_SIZE counters)
 """
 from __future__ import annotations
 
@@ -133,8 +136,11 @@ def cpu_baseline(sizes=(1024, 2048)):
 
 
 def branching_leg(lib):
-    """128 strong-branch child LPs of the n=500, m=200 knapsack root: batched vs per-LP."""
-    from sovereign_amd import api, milp, problems as P
+    """Strong branching at the root of the n=500, m=200 knapsack: the 128
+    child LPs of 64 candidates batched vs per-LP (same scores), and one
+    branch-and-bound node processed three ways (per-LP full strong branching,
+    batched full, batched on the surrogate's top 8)."""
+    from sovereign_amd import api, milp, problems as P, surrogate as S
 
     model = P.multi_knapsack(500, 200, seed=P.BASE_SEED)
     is_int = np.ones(model.n_vars, dtype=np.int8)
@@ -163,6 +169,28 @@ def branching_leg(lib):
     })
     log(f"strong branching: {children} child LPs batched {wall_b:.3f}s ({kms:.1f} ms kernel), per-LP "
         f"{wall_p:.3f}s, same choice {best_b == best_p}, max rel diff {diff:.2e}")
+    # one branch-and-bound node (the root), three ways
+    node = {}
+    ranker = S.load_trained(model) if os.path.exists(S.WEIGHTS) else None
+    setups = [("per_lp_full_64", dict(strong_branch_candidates=64, child_solver=milp.CHILD_PER_LP)),
+              ("batched_full_64", dict(strong_branch_candidates=64))]
+    if ranker is not None:
+        setups.append(("batched_surrogate_top8", dict(strong_branch_candidates=8, ranker=ranker)))
+    for name, kw in setups:
+        t0 = time.perf_counter()
+        r = milp.BranchAndBoundSolver(max_nodes=1, **kw).solve(model, is_int)
+        node[name] = {"node_wall_s": time.perf_counter() - t0, "strong_branch_s": r["strong_branch_seconds"],
+                      "lp_solves": r["lp_solves"]}
+        log(f"root node {name}: {node[name]['node_wall_s']:.3f}s (strong branching {r['strong_branch_seconds']:.3f}s, "
+            f"{r['lp_solves']} LP solves)")
+    out["bnb_root_node"] = node
+    if ranker is not None:
+        ys = ranker.scores(root["primal"], model.lb, model.ub, [c["var"] for c in cb])
+        top8 = set(np.argsort(-ys, kind="stable")[:8])
+        out["surrogate_top8_contains_full_sb_choice"] = bool(best_b in top8)
+        out["wall_per_node_speedup_surrogate_vs_per_lp_full"] = (
+            node["per_lp_full_64"]["node_wall_s"] / node["batched_surrogate_top8"]["node_wall_s"])
+        ranker.net.close()
     return out
 
 
@@ -176,7 +204,19 @@ def main():
     ap.add_argument("--paths", default="mixed,fp64")
     ap.add_argument("--m", type=int, default=8192)
     ap.add_argument("--n", type=int, default=32768)
+    ap.add_argument("--copy-probe", action="store_true")
     args = ap.parse_args()
+    if args.copy_probe:
+        import torch
+
+        x = torch.ones(64 * 1024 * 1024, device="cuda", dtype=torch.float32)
+        y = torch.empty_like(x)
+        torch.cuda.synchronize()
+        for _ in range(3):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes": x.numel() * 4, "copies": 3}))
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (the sharding design is in DESIGN.md).",
               file=sys.stderr)
