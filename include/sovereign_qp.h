@@ -1,0 +1,66 @@
+/* sovereign_qp.h - the structured separable convex QP on the GPU (C ABI).
+ *
+ *     min 0.5 ||x||^2 - target' x   s.t.  x_r - 2 x_{r+1} + x_{r+2} >= 0,
+ *                                          r = 0 .. n-3, x free
+ *
+ * This is the LISWET shape that the reference's QP solver detects
+ * (detect_liswet_structured, solver/qp/src/qp_interior_point.cpp:892-967)
+ * and solves through ConvexQpSolver::solve -> solve_liswet_structured_gi /
+ * solve_liswet_structured_ipm (:3153-3161). sov_qp_liswet_solve replaces
+ * that dispatch for this shape: one call takes the target vector (the
+ * negated linear term of the model) and returns x, the multipliers and the
+ * certificate that certify_liswet (:1410-1480) computes.
+ *
+ * Method: the reference's x-space interior-point iterations
+ * (solve_liswet_structured_ipm, :2378-2826) in one persistent GPU workgroup,
+ * the pentadiagonal normal matrix I + A' diag(z/s) A solved by block cyclic
+ * reduction of its 2 x 2-block tridiagonal form, factored in FP32 with FP64
+ * refinement (SOV_PREC_FP32) or in FP64 throughout (SOV_PREC_FP64), then the
+ * knot-space polish of polish_liswet_knots (:2269-2377) and the certificate
+ * on the host in long double. Status codes as in sovereign_hip.h. */
+#ifndef SOVEREIGN_QP_H
+#define SOVEREIGN_QP_H
+
+#include "sovereign_hip.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct {
+  int32_t max_iterations; /* IPM iterations (reference default 200) */
+  int32_t precision;      /* SOV_PREC_FP32 (mixed) or SOV_PREC_FP64 */
+  double tolerance;       /* <= 0: 1e-10, 1e-12 when n - 2 <= 50 (the reference's structured tolerances) */
+  int32_t polish;         /* 1: knot-space polish from the last iterate (default) */
+  int32_t pad;
+} sov_qp_options;
+
+typedef struct {
+  int32_t status;      /* 0: certified optimal; 6: the certificate failed */
+  int32_t ipm_status;  /* 0 reached the tolerance, 1 stopped (iteration cap), 6 numerical failure */
+  int32_t iterations;
+  int32_t polish_passes;
+  int32_t fp32_factorizations, fp64_factorizations;
+  int32_t refinements;
+  int32_t pad;
+  double objective;    /* 0.5 |x|^2 - target' x */
+  double primal;       /* max(0, -(A x)_r) */
+  double stationarity; /* |x - target - A' lambda|_inf */
+  double complementarity, min_lambda, consistency;
+  double ipm_primal, ipm_dual, ipm_complementarity; /* scaled residuals of the last iterate */
+  double kernel_ms;    /* GPU time of the interior-point launch */
+  double seconds;      /* wall time of the call */
+} sov_qp_result;
+
+void sov_qp_default_options(sov_qp_options* opt);
+
+/* Solves the QP for target[0..n) (n >= 3). x_out (n) and lambda_out (n - 2)
+ * may be NULL. opt NULL means the defaults. */
+int sov_qp_liswet_solve(int32_t n, const double* target, const sov_qp_options* opt, sov_qp_result* res,
+                        double* x_out, double* lambda_out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SOVEREIGN_QP_H */

@@ -73,6 +73,8 @@ _SIGS = {
     "orc_amd_order": (C.c_int, [C.c_int32, _i32p, _i32p, C.c_int64, _i32p, _i32p]),
     "orc_ldlt_factor_solve": (C.c_int, [C.c_int32, _i32p, _i32p, _f64p, C.c_double, C.c_double, _f64p, _i32p,
                                         _i64p, _i32p]),
+    "orc_liswet_ipm": (C.c_int, [C.c_int32, _f64p, C.c_int32, C.c_int32, _f64p]),
+    "orc_liswet_certify": (C.c_int, [C.c_int32, _f64p, _f64p, _f64p, _f64p]),
 }
 
 _REF_SIGS = {
@@ -304,3 +306,34 @@ def ref_sparse_normal_solve(m, n, col_ptr, row_idx, values, d, rhs):
     st = ref().ref_sparse_normal_solve(m, n, pi(i32(col_ptr)), pi(i32(row_idx)), pf(f64(values)), pf(f64(d)), pf(x),
                                        C.byref(lnnz))
     return st, x, lnnz.value
+
+
+LISWET_MODES = {"tolerance": 0, "reference_default": 1, "polish_last": 2}
+
+
+def liswet_ipm(target, max_iterations=200, mode="polish_last"):
+    """The reference's structured QP IPM restated (liswet_oracle.cpp).
+
+    mode "tolerance": iterate to the structured tolerance (1e-10, 1e-12 for
+    m <= 50) and return the last iterate; "reference_default": the
+    reference's default (knot-space polish from complementarity 1e-6,
+    :2473-2477); "polish_last": iterate to the tolerance or the iteration cap,
+    then polish from the last iterate. Returns (iterations or a negative
+    failure code: -2 method failure, -3 polish failure, x)."""
+    t = f64(target)
+    x = np.zeros(len(t))
+    it = lib().orc_liswet_ipm(len(t), pf(t), max_iterations, LISWET_MODES[mode], pf(x))
+    return it, x
+
+
+def liswet_certify(target, x):
+    """certify_liswet restated: dict of objective, primal, stationarity,
+    complementarity, min_lambda, consistency, ok; and lambda."""
+    t, xv = f64(target), f64(x)
+    out = np.zeros(7)
+    lam = np.zeros(len(t) - 2)
+    lib().orc_liswet_certify(len(t), pf(t), pf(xv), pf(lam), pf(out))
+    keys = ("objective", "primal", "stationarity", "complementarity", "min_lambda", "consistency", "ok")
+    d = dict(zip(keys, out.tolist()))
+    d["ok"] = bool(d["ok"])
+    return d, lam

@@ -131,6 +131,13 @@ int orc_ldlt_factor_solve(int32_t n, const int32_t* p_col_ptr, const int32_t* p_
 int orc_amd_order(int32_t n, const int32_t* p_col_ptr, const int32_t* p_row_idx,
                   int64_t fill_limit, int32_t* perm, int32_t* iperm);
 
+/* Structured separable QP  min 0.5|x|^2 - target'x,  x_r - 2x_{r+1} + x_{r+2} >= 0
+ * (liswet_oracle.cpp; solve_liswet_structured_ipm, qp_interior_point.cpp:2378-2826,
+ * and certify_liswet, :1410-1480). */
+int orc_liswet_ipm(int32_t n, const double* target, int32_t max_iterations, int32_t mode,
+                   double* x_out);
+int orc_liswet_certify(int32_t n, const double* target, const double* x, double* lambda_out, double* out7);
+
 #ifdef __cplusplus
 }
 #endif

@@ -140,6 +140,25 @@ class MlpShape(C.Structure):
     _fields_ = [("in_", C.c_int32), ("hidden1", C.c_int32), ("hidden2", C.c_int32), ("pad", C.c_int32)]
 
 
+class QpOptions(C.Structure):
+    _fields_ = [("max_iterations", C.c_int32), ("precision", C.c_int32), ("tolerance", C.c_double),
+                ("polish", C.c_int32), ("pad", C.c_int32)]
+
+
+class QpResult(C.Structure):
+    _fields_ = [
+        ("status", C.c_int32), ("ipm_status", C.c_int32), ("iterations", C.c_int32), ("polish_passes", C.c_int32),
+        ("fp32_factorizations", C.c_int32), ("fp64_factorizations", C.c_int32), ("refinements", C.c_int32),
+        ("pad", C.c_int32), ("objective", C.c_double), ("primal", C.c_double), ("stationarity", C.c_double),
+        ("complementarity", C.c_double), ("min_lambda", C.c_double), ("consistency", C.c_double),
+        ("ipm_primal", C.c_double), ("ipm_dual", C.c_double), ("ipm_complementarity", C.c_double),
+        ("kernel_ms", C.c_double), ("seconds", C.c_double),
+    ]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_ if name != "pad"}
+
+
 _f32p = C.POINTER(C.c_float)
 
 SIGNATURES = {
@@ -194,6 +213,8 @@ SIGNATURES = {
     "sov_mlp_train": (C.c_int, [_vp, _f32p, _f32p, C.c_int64, C.c_int32, C.c_int32, C.c_float, C.c_uint64,
                                 _f32p]),
     "sov_mlp_free": (C.c_int, [_vp]),
+    "sov_qp_default_options": (None, [C.POINTER(QpOptions)]),
+    "sov_qp_liswet_solve": (C.c_int, [C.c_int32, _f64p, C.POINTER(QpOptions), C.POINTER(QpResult), _f64p, _f64p]),
 }
 
 _lib = None

@@ -1,0 +1,156 @@
+"""The structured separable convex QP on the GPU, over the C ABI
+(include/sovereign_qp.h).
+
+    min 0.5 |x|^2 - target'x   s.t.  x_r - 2 x_{r+1} + x_{r+2} >= 0,  x free
+
+Mirrors the reference's QP entry point for this shape:
+  QpInteriorPointOptions / QpInteriorPointSolver(options).solve(model)
+      solver/qp/include/sovereign/qp_interior_point.hpp:7-31
+  detect_liswet_structured(model)      solver/qp/src/qp_interior_point.cpp:892-967
+  the structured dispatch              qp_interior_point.cpp:3153-3161
+  certify_liswet                       qp_interior_point.cpp:1410-1480
+
+A model that is not of this shape gets status NotImplemented: the generic
+sparse QP interior-point path of the reference is not part of this library.
+Everything runs through libsovereign_hip.so; there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _capi as K
+from .api import STATUS_NAMES, _require_device
+
+INF = 1e30
+
+
+@dataclass
+class QpModel:
+    """A continuous QP in the reference's model terms (types.hpp
+    OptimizationModel, restricted to what the structured detector reads):
+    minimize 0.5 x'Qx + linear'x over variables with bounds [lb, ub],
+    rows sum_j coef_j x_j (sense) rhs. quadratic maps (i, j) -> Q_ij, rows
+    are dicts {var index: coefficient}, senses '>=', '<=' or '='."""
+
+    n_vars: int
+    linear: np.ndarray
+    quadratic: dict
+    rows: list
+    senses: list
+    rhs: np.ndarray
+    lb: np.ndarray = field(default=None)
+    ub: np.ndarray = field(default=None)
+    minimize: bool = True
+
+    def __post_init__(self):
+        self.linear = np.asarray(self.linear, dtype=np.float64)
+        self.rhs = np.asarray(self.rhs, dtype=np.float64)
+        if self.lb is None:
+            self.lb = np.full(self.n_vars, -INF)
+        if self.ub is None:
+            self.ub = np.full(self.n_vars, INF)
+
+
+def liswet_model(target) -> QpModel:
+    """The structured QP for `target` as a model: Q = I, linear = -target,
+    rows x_r - 2 x_{r+1} + x_{r+2} >= 0, free variables."""
+    t = np.asarray(target, dtype=np.float64)
+    n = len(t)
+    return QpModel(n_vars=n, linear=-t, quadratic={(i, i): 1.0 for i in range(n)},
+                   rows=[{r: 1.0, r + 1: -2.0, r + 2: 1.0} for r in range(n - 2)],
+                   senses=[">="] * (n - 2), rhs=np.zeros(n - 2))
+
+
+def detect_liswet(model: QpModel):
+    """The target vector when the model has the structured shape, else None
+    (the acceptance rules of detect_liswet_structured, :892-967: a
+    minimization, n >= 3 free variables, n - 2 rows, Q = I exactly (1e-12),
+    row r = x_r - 2 x_{r+1} + x_{r+2} >= 0 with rhs 0 (1e-12))."""
+    n = model.n_vars
+    if not model.minimize or n < 3 or len(model.rows) != n - 2:
+        return None
+    if np.any(model.lb > -1e29) or np.any(np.isfinite(model.ub) & (model.ub < 1e29)):
+        return None
+    diag = np.zeros(n, dtype=bool)
+    for (i, j), v in model.quadratic.items():
+        if i != j or abs(v - 1.0) > 1e-12:
+            return None
+        diag[i] = True
+    if not diag.all():
+        return None
+    expected = (1.0, -2.0, 1.0)
+    for r, (row, sense) in enumerate(zip(model.rows, model.senses)):
+        if sense != ">=" or abs(model.rhs[r]) > 1e-12 or len(row) != 3:
+            return None
+        for k in range(3):
+            if r + k not in row or abs(row[r + k] - expected[k]) > 1e-12:
+                return None
+    return -model.linear.copy()
+
+
+@dataclass
+class QpInteriorPointOptions:
+    """qp_interior_point.hpp:7-18 (the fields the structured path reads),
+    plus the precision policy of the GPU path."""
+    max_iterations: int = 200
+    precision: str = "mixed"  # 'mixed': FP32 factor + FP64 refinement; 'fp64': FP64 throughout
+    tolerance: float = 0.0    # 0: the structured tolerances (1e-10; 1e-12 when rows <= 50)
+    polish: bool = True
+
+    def to_c(self) -> K.QpOptions:
+        o = K.QpOptions()
+        o.max_iterations = self.max_iterations
+        o.precision = K.SOV_PREC_FP32 if self.precision == "mixed" else K.SOV_PREC_FP64
+        o.tolerance = self.tolerance
+        o.polish = 1 if self.polish else 0
+        return o
+
+
+def solve_liswet(target, options: QpInteriorPointOptions | None = None) -> dict:
+    """Solves the structured QP for `target`; dict with status, objective,
+    x, lam (multipliers), the certificate and the interior-point record."""
+    lib = K.load()
+    _require_device()
+    t = K.f64(target)
+    n = len(t)
+    if n < 3:
+        raise ValueError("the structured QP needs at least 3 variables")
+    x = np.zeros(n)
+    lam = np.zeros(n - 2)
+    res = K.QpResult()
+    opt = (options or QpInteriorPointOptions()).to_c()
+    rc = lib.sov_qp_liswet_solve(n, K.pf(t), C.byref(opt), C.byref(res), K.pf(x), K.pf(lam))
+    K.check(rc, "sov_qp_liswet_solve")
+    out = res.as_dict()
+    out["status_name"] = STATUS_NAMES.get(res.status, str(res.status))
+    out.update(x=x, lam=lam)
+    return out
+
+
+class QpInteriorPointSolver:
+    """QpInteriorPointSolver (qp_interior_point.hpp:24-31) for the structured
+    shape. solve(QpModel) -> dict (status 0 Optimal with a certificate, 6
+    NumericalError when the certificate fails, 8 NotImplemented for any other
+    shape)."""
+
+    def __init__(self, options: QpInteriorPointOptions | None = None):
+        self.options = options or QpInteriorPointOptions()
+
+    def solve(self, model: QpModel) -> dict:
+        target = detect_liswet(model)
+        if target is None:
+            return {"status": 8, "status_name": STATUS_NAMES[8],
+                    "message": "only the structured separable QP (second-difference rows) runs on the GPU"}
+        return solve_liswet(target, self.options)
+
+
+def liswet_target(n: int = 20002, seed: int = 20260501, noise_sd: float = 0.01) -> np.ndarray:
+    """Synthetic target of the n-variable workload: t_i = i / (n - 1),
+    target = sin(3t) + t^2 + e, e ~ Normal(0, noise_sd) (noise_sd is a
+    standard deviation)."""
+    rng = np.random.default_rng(seed)
+    t = np.arange(n, dtype=np.float64) / (n - 1)
+    return np.sin(3.0 * t) + t * t + rng.normal(0.0, noise_sd, n)
