@@ -27,8 +27,12 @@ path (minimum-degree ordered sparse LDL^T, one thread) on two bounded
 samples of the same generator (m = 1024 and 2048), extrapolated to m = 8192
 with the exponent fitted between them.
 
+Separable QP: 20,002 variables, 20,000 second-difference rows, solved on
+the mixed and the plain FP64 path (synthetic target and the reference's own
+n=20000 instance), with the oracle's one-thread solve as its CPU baseline.
+
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
-                       [--paths mixed,fp64] [--no-branching]
+                       [--paths mixed,fp64] [--no-branching] [--no-qp]
        python bench.py --copy-probe   (a plain 256 MiB device copy, three
                                        times: the calibration run for the
                                        FETCH_SIZE / WRITE_SIZE counters)
@@ -194,6 +198,60 @@ def branching_leg(lib):
     return out
 
 
+def qp_leg(steps, with_cpu):
+    """The separable convex QP (20,002 variables, 20,000 second-difference
+    rows): the synthetic workload (target = sin(3t) + t^2 + Normal(0, 0.01)
+    noise, seed 20260501) and the reference's own n=20000 instance
+    (tests/golden), each solved `steps` times on the mixed and the plain FP64
+    path. Time = the interior-point kernel (HIP events; one launch per solve,
+    inputs resident) and the whole call (upload, kernel, host polish and
+    certificate). Per-iteration algorithmic traffic: 64 bytes per row."""
+    from sovereign_amd import qp
+
+    fx = np.load(os.path.join(ROOT, "tests", "golden", "data", "liswet_fixtures.npz"))
+    with open(os.path.join(ROOT, "tests", "golden", "liswet_golden.json")) as fh:
+        gold = float(json.load(fh)["cases"]["20000"]["objective"])
+    cases = {"synthetic_n20002": qp.liswet_target(20002), "reference_n20000": fx["target_20000"]}
+    out = {"model": "separable convex QP, n=20002 vars, m=20000 rows x_i - 2x_{i+1} + x_{i+2} >= 0, Q = I",
+           "reference_cpu_gi_s_n20000": 180.9}
+    for name, target in cases.items():
+        rec = {}
+        for prec in ("mixed", "fp64"):
+            opts = qp.QpInteriorPointOptions(precision=prec)
+            qp.solve_liswet(target, opts)  # warm-up
+            runs = [qp.solve_liswet(target, opts) for _ in range(steps)]
+            r = runs[-1]
+            kms = sorted(x["kernel_ms"] for x in runs)[len(runs) // 2]
+            wall = sorted(x["seconds"] for x in runs)[len(runs) // 2]
+            bytes_alg = 64.0 * (len(target) - 2) * r["iterations"]
+            rec[prec] = {"status": r["status"], "ipm_iterations": r["iterations"], "ipm_status": r["ipm_status"],
+                         "fp32_factorizations": r["fp32_factorizations"],
+                         "fp64_factorizations": r["fp64_factorizations"], "polish_passes": r["polish_passes"],
+                         "objective": r["objective"], "kernel_ms": kms, "call_s": wall,
+                         "kernel_algorithmic_GBps": bytes_alg / (kms * 1e-3) / 1e9}
+            log(f"qp {name} {prec}: status {r['status']} obj {r['objective']:.15g} kernel {kms:.1f} ms, "
+                f"call {wall * 1e3:.1f} ms, {r['iterations']} iterations")
+        rec["speedup_mixed_vs_fp64_kernel"] = rec["fp64"]["kernel_ms"] / rec["mixed"]["kernel_ms"]
+        if name == "reference_n20000":
+            rec["objective_rel_diff_vs_reference"] = abs(rec["fp64"]["objective"] - gold) / abs(gold)
+        out[name] = rec
+    if with_cpu:
+        from oracle import oracle as O
+
+        t0 = time.perf_counter()
+        it, xo = O.liswet_ipm(cases["synthetic_n20002"])
+        cpu_s = time.perf_counter() - t0
+        cert, _ = O.liswet_certify(cases["synthetic_n20002"], xo)
+        out["cpu_baseline"] = {"value": cpu_s, "unit": "s per QP solve (synthetic n=20002)", "cores": 1,
+                               "kind": "port",
+                               "sample": "oracle restatement of the reference structured QP IPM (long double, "
+                                         "one thread) plus knot polish, one full solve"}
+        out["synthetic_n20002"]["objective_rel_diff_vs_oracle"] = (
+            abs(out["synthetic_n20002"]["fp64"]["objective"] - cert["objective"]) / abs(cert["objective"]))
+        log(f"qp cpu oracle {cpu_s:.2f}s obj {cert['objective']:.15g}")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -201,6 +259,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-branching", action="store_true")
+    ap.add_argument("--no-qp", action="store_true")
     ap.add_argument("--paths", default="mixed,fp64")
     ap.add_argument("--m", type=int, default=8192)
     ap.add_argument("--n", type=int, default=32768)
@@ -288,6 +347,8 @@ def main():
         })
     if not args.no_branching:
         line["strong_branching"] = branching_leg(lib)
+    if not args.no_qp:
+        line["separable_qp"] = qp_leg(max(3, args.steps), not args.no_cpu)
     if not args.no_cpu:
         log("cpu baseline (oracle, m=1024 and m=2048)")
         line["cpu_baseline"] = cpu_baseline()

@@ -48,6 +48,8 @@ typedef struct {
   double stationarity; /* |x - target - A' lambda|_inf */
   double complementarity, min_lambda, consistency;
   double ipm_primal, ipm_dual, ipm_complementarity; /* scaled residuals of the last iterate */
+  double factor_ms;    /* part of kernel_ms spent in factorizations */
+  double solve_ms;     /* part of kernel_ms spent in factor solves */
   double kernel_ms;    /* GPU time of the interior-point launch */
   double seconds;      /* wall time of the call */
 } sov_qp_result;

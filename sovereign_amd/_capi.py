@@ -152,7 +152,7 @@ class QpResult(C.Structure):
         ("pad", C.c_int32), ("objective", C.c_double), ("primal", C.c_double), ("stationarity", C.c_double),
         ("complementarity", C.c_double), ("min_lambda", C.c_double), ("consistency", C.c_double),
         ("ipm_primal", C.c_double), ("ipm_dual", C.c_double), ("ipm_complementarity", C.c_double),
-        ("kernel_ms", C.c_double), ("seconds", C.c_double),
+        ("factor_ms", C.c_double), ("solve_ms", C.c_double), ("kernel_ms", C.c_double), ("seconds", C.c_double),
     ]
 
     def as_dict(self):

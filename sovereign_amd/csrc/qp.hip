@@ -52,6 +52,7 @@ struct QpDev {
   int refinements;
   int pad;
   double pres, dres, cres, mu;
+  long long t_factor, t_solve;  // wall-clock ticks spent in factorizations and in solves
 };
 
 // Reduction blocks of one factor (global memory), arithmetic type T.
@@ -359,6 +360,22 @@ struct Ctx {
   int ok;
 };
 
+// Factor solve with its time booked to the result record (thread 0).
+__device__ void timed_solve(int n, Ctx& c, const double* v, double* out) {
+  const long long t0 = wall_clock64();
+  if (c.prec == SOV_PREC_FP32) solve<float>(n, c.w, c.L, c.f32, v, out);
+  else solve<double>(n, c.w, c.L, c.f64, v, out);
+  if (threadIdx.x == 0) c.rec.t_solve += wall_clock64() - t0;
+}
+
+template <typename T>
+__device__ bool timed_factor(int n, Ctx& c, double reg, const Fac<T>& f) {
+  const long long t0 = wall_clock64();
+  const bool ok = factor<T>(n, c.w, c.L, reg, f, &c.ok);
+  if (threadIdx.x == 0) c.rec.t_factor += wall_clock64() - t0;
+  return ok;
+}
+
 // Solves (M + reg I) dx = rhs with the current factor, refining against the
 // FP64 bands. FP64 factor: at most two corrections, as the reference. FP32
 // factor: at most eight; false when the residual is not below 1e-10 x |rhs|
@@ -367,8 +384,7 @@ __device__ bool refined_solve(int n, Ctx& c, double* dx) {
   const Ws& w = c.w;
   const int tid = threadIdx.x;
   const bool fp32 = c.prec == SOV_PREC_FP32;
-  if (fp32) solve<float>(n, w, c.L, c.f32, w.rhs, dx);
-  else solve<double>(n, w, c.L, c.f64, w.rhs, dx);
+  timed_solve(n, c, w.rhs, dx);
   double rn0 = 0.0;
   for (int j = tid; j < n; j += kT) rn0 = fmax(rn0, fabs(w.rhs[j]));
   rn0 = fmax(bmax(rn0, c.sh), 1e-300);
@@ -386,8 +402,7 @@ __device__ bool refined_solve(int n, Ctx& c, double* dx) {
     if (!isfinite(rn)) return false;
     if (rn <= 1e-14 * rn0) return true;
     if (k == max_ref) return !fp32 || rn <= 1e-10 * rn0;
-    if (fp32) solve<float>(n, w, c.L, c.f32, w.res, w.corr);
-    else solve<double>(n, w, c.L, c.f64, w.res, w.corr);
+    timed_solve(n, c, w.res, w.corr);
     for (int j = tid; j < n; j += kT) dx[j] += w.corr[j];
     if (tid == 0) ++c.rec.refinements;
     __syncthreads();
@@ -428,7 +443,7 @@ __device__ bool direction_safe(int n, Ctx& c, double* dx, double* ds, double* dz
   __syncthreads();
   if (threadIdx.x == 0) c.prec = SOV_PREC_FP64;
   __syncthreads();
-  if (!factor<double>(n, c.w, c.L, c.reg, c.f64, &c.ok)) return false;
+  if (!timed_factor<double>(n, c, c.reg, c.f64)) return false;
   if (threadIdx.x == 0) ++c.rec.fp64_factors;
   return direction(n, c, dx, ds, dz);
 }
@@ -536,7 +551,7 @@ __global__ __launch_bounds__(kT) void liswet_ipm_kernel(int n, const double* __r
       if (tid == 0) c.reg = reg;
       __syncthreads();
       if (c.prec == SOV_PREC_FP32) {
-        fok = factor<float>(n, w, c.L, reg, c.f32, &c.ok);
+        fok = timed_factor<float>(n, c, reg, c.f32);
         if (fok) {
           if (tid == 0) ++c.rec.fp32_factors;
           break;
@@ -545,7 +560,7 @@ __global__ __launch_bounds__(kT) void liswet_ipm_kernel(int n, const double* __r
         if (tid == 0) c.prec = SOV_PREC_FP64;
         __syncthreads();
       }
-      fok = factor<double>(n, w, c.L, reg, c.f64, &c.ok);
+      fok = timed_factor<double>(n, c, reg, c.f64);
       if (fok && tid == 0) ++c.rec.fp64_factors;
     }
     if (!fok) {
@@ -890,6 +905,14 @@ int sov_qp_liswet_solve(int32_t n, const double* target, const sov_qp_options* o
   res->ipm_dual = rec.dres;
   res->ipm_complementarity = rec.cres;
   res->kernel_ms = kms;
+  {
+    int dev = 0, khz = 0;
+    if (hipGetDevice(&dev) == hipSuccess &&
+        hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev) == hipSuccess && khz > 0) {
+      res->factor_ms = static_cast<double>(rec.t_factor) / khz;
+      res->solve_ms = static_cast<double>(rec.t_solve) / khz;
+    }
+  }
   // knot-space polish and certificate on the host
   std::vector<ld> t(target, target + n), x(xh.begin(), xh.end()), sv(sh.begin(), sh.end()), zv(zh.begin(), zh.end());
   Certificate cert;
