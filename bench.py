@@ -226,7 +226,7 @@ def qp_leg(steps, with_cpu):
             bytes_alg = 64.0 * (len(target) - 2) * r["iterations"]
             rec[prec] = {"status": r["status"], "ipm_iterations": r["iterations"], "ipm_status": r["ipm_status"],
                          "fp32_factorizations": r["fp32_factorizations"],
-                         "fp64_factorizations": r["fp64_factorizations"], "polish_passes": r["polish_passes"],
+                         "fp64_factorizations": r["fp64_factorizations"], "polish_fits": r["polish_fits"],
                          "objective": r["objective"], "kernel_ms": kms, "call_s": wall,
                          "kernel_algorithmic_GBps": bytes_alg / (kms * 1e-3) / 1e9}
             log(f"qp {name} {prec}: status {r['status']} obj {r['objective']:.15g} kernel {kms:.1f} ms, "

@@ -15,9 +15,13 @@
  * (solve_liswet_structured_ipm, :2378-2826) in one persistent GPU workgroup,
  * the pentadiagonal normal matrix I + A' diag(z/s) A solved by block cyclic
  * reduction of its 2 x 2-block tridiagonal form, factored in FP32 with FP64
- * refinement (SOV_PREC_FP32) or in FP64 throughout (SOV_PREC_FP64), then the
- * knot-space polish of polish_liswet_knots (:2269-2377) and the certificate
- * on the host in long double. Status codes as in sovereign_hip.h. */
+ * refinement (SOV_PREC_FP32) or in FP64 throughout (SOV_PREC_FP64), then a
+ * knot-space polish on the host in long double: a primal active-set method
+ * over the rows' active set, warm-started from the interior point's guess
+ * {r : s_r < z_r} as polish_liswet_knots (:2269-2377) is, each candidate the
+ * least-squares piecewise-linear fit of solve_liswet_knot_projection
+ * (:1509-1580), and the result certified by certify_liswet's checks.
+ * Status codes as in sovereign_hip.h. */
 #ifndef SOVEREIGN_QP_H
 #define SOVEREIGN_QP_H
 
@@ -39,7 +43,7 @@ typedef struct {
   int32_t status;      /* 0: certified optimal; 6: the certificate failed */
   int32_t ipm_status;  /* 0 reached the tolerance, 1 stopped (iteration cap), 6 numerical failure */
   int32_t iterations;
-  int32_t polish_passes;
+  int32_t polish_fits; /* knot fits computed by the polish */
   int32_t fp32_factorizations, fp64_factorizations;
   int32_t refinements;
   int32_t pad;
@@ -60,6 +64,14 @@ void sov_qp_default_options(sov_qp_options* opt);
  * may be NULL. opt NULL means the defaults. */
 int sov_qp_liswet_solve(int32_t n, const double* target, const sov_qp_options* opt, sov_qp_result* res,
                         double* x_out, double* lambda_out);
+
+/* The host polish alone, from an interior point (s, z: n - 2 entries each)
+ * of the same QP: the polish step of the solver above (the role of
+ * polish_liswet_knots, :2269-2377), for callers that run their own
+ * interior-point iterations. Needs no device. Fills status, polish_fits,
+ * objective, the certificate fields and seconds of res. */
+int sov_qp_liswet_polish(int32_t n, const double* target, const double* s, const double* z, sov_qp_result* res,
+                         double* x_out, double* lambda_out);
 
 #ifdef __cplusplus
 }

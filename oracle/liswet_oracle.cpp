@@ -315,6 +315,17 @@ int orc_liswet_certify(int32_t n, const double* target, const double* x_in, doub
   return 0;
 }
 
+// Knot-space polish from an interior point (s, z): passes used (> 0) or -1;
+// x_out receives the certified point.
+int orc_liswet_polish(int32_t n, const double* target, const double* s_in, const double* z_in, double* x_out) {
+  if (n < 3 || !target || !s_in || !z_in || !x_out) return -2;
+  const std::size_t N = static_cast<std::size_t>(n), m = N - 2;
+  std::vector<ld> t(target, target + N), s(s_in, s_in + m), z(z_in, z_in + m), x;
+  if (!polish(t, s, z, x)) return -1;
+  for (std::size_t j = 0; j < N; ++j) x_out[j] = static_cast<double>(x[j]);
+  return 1;
+}
+
 // The structured IPM in mode 0, 1 or 2 (above). Returns iterations (>= 0),
 // -1 on bad input, -2 when the method failed (factorization, non-finite
 // step), -3 when the polish failed; x_out gets the last iterate either way.

@@ -75,6 +75,7 @@ _SIGS = {
                                         _i64p, _i32p]),
     "orc_liswet_ipm": (C.c_int, [C.c_int32, _f64p, C.c_int32, C.c_int32, _f64p]),
     "orc_liswet_certify": (C.c_int, [C.c_int32, _f64p, _f64p, _f64p, _f64p]),
+    "orc_liswet_polish": (C.c_int, [C.c_int32, _f64p, _f64p, _f64p, _f64p]),
 }
 
 _REF_SIGS = {
@@ -324,6 +325,15 @@ def liswet_ipm(target, max_iterations=200, mode="polish_last"):
     x = np.zeros(len(t))
     it = lib().orc_liswet_ipm(len(t), pf(t), max_iterations, LISWET_MODES[mode], pf(x))
     return it, x
+
+
+def liswet_polish(target, s, z):
+    """polish_liswet_knots restated: the certified x from the interior point
+    (s, z), or None when the polish fails."""
+    t = f64(target)
+    x = np.zeros(len(t))
+    rc = lib().orc_liswet_polish(len(t), pf(t), pf(f64(s)), pf(f64(z)), pf(x))
+    return x if rc > 0 else None
 
 
 def liswet_certify(target, x):

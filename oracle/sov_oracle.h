@@ -137,6 +137,7 @@ int orc_amd_order(int32_t n, const int32_t* p_col_ptr, const int32_t* p_row_idx,
 int orc_liswet_ipm(int32_t n, const double* target, int32_t max_iterations, int32_t mode,
                    double* x_out);
 int orc_liswet_certify(int32_t n, const double* target, const double* x, double* lambda_out, double* out7);
+int orc_liswet_polish(int32_t n, const double* target, const double* s, const double* z, double* x_out);
 
 #ifdef __cplusplus
 }

@@ -147,7 +147,7 @@ class QpOptions(C.Structure):
 
 class QpResult(C.Structure):
     _fields_ = [
-        ("status", C.c_int32), ("ipm_status", C.c_int32), ("iterations", C.c_int32), ("polish_passes", C.c_int32),
+        ("status", C.c_int32), ("ipm_status", C.c_int32), ("iterations", C.c_int32), ("polish_fits", C.c_int32),
         ("fp32_factorizations", C.c_int32), ("fp64_factorizations", C.c_int32), ("refinements", C.c_int32),
         ("pad", C.c_int32), ("objective", C.c_double), ("primal", C.c_double), ("stationarity", C.c_double),
         ("complementarity", C.c_double), ("min_lambda", C.c_double), ("consistency", C.c_double),
@@ -215,6 +215,7 @@ SIGNATURES = {
     "sov_mlp_free": (C.c_int, [_vp]),
     "sov_qp_default_options": (None, [C.POINTER(QpOptions)]),
     "sov_qp_liswet_solve": (C.c_int, [C.c_int32, _f64p, C.POINTER(QpOptions), C.POINTER(QpResult), _f64p, _f64p]),
+    "sov_qp_liswet_polish": (C.c_int, [C.c_int32, _f64p, _f64p, _f64p, C.POINTER(QpResult), _f64p, _f64p]),
 }
 
 _lib = None

@@ -25,6 +25,7 @@
 
 #include "internal.hpp"
 
+#include "qp_polish.hpp"
 #include "sovereign_qp.h"
 
 #include <algorithm>
@@ -605,187 +606,7 @@ __global__ __launch_bounds__(kT) void liswet_ipm_kernel(int n, const double* __r
   if (tid == 0) *out = c.rec;
 }
 
-// ---- host: knot-space polish and certificate (long double) ----------------
-
-typedef long double ld;
-
-void h_apply_a(const std::vector<ld>& x, std::vector<ld>& out) {
-  const std::size_t m = x.size() - 2;
-  out.assign(m, 0.0L);
-  for (std::size_t r = 0; r < m; ++r) out[r] = x[r] - 2.0L * x[r + 1] + x[r + 2];
-}
-
-void h_apply_at(const std::vector<ld>& z, std::size_t n, std::vector<ld>& out) {
-  out.assign(n, 0.0L);
-  for (std::size_t r = 0; r < z.size(); ++r) {
-    out[r] += z[r];
-    out[r + 1] -= 2.0L * z[r];
-    out[r + 2] += z[r];
-  }
-}
-
-struct Certificate {
-  ld primal = 0, stationarity = 0, complementarity = 0, min_lambda = 0, consistency = 0;
-  bool ok = false;
-};
-
-// certify_liswet (qp_interior_point.cpp:1410-1480): multipliers from the
-// second-difference recurrence of x - target, then primal feasibility, dual
-// sign, stationarity, consistency of the last two rows and complementarity,
-// each within 1e-8 x max(1, |target|, |A' lambda|, |x|).
-Certificate certify(const std::vector<ld>& t, const std::vector<ld>& x, std::vector<ld>& lam,
-                    std::vector<ld>& slack) {
-  const std::size_t N = t.size(), m = N - 2;
-  std::vector<ld> res(N), atr;
-  lam.assign(m, 0.0L);
-  for (std::size_t j = 0; j < N; ++j) res[j] = x[j] - t[j];
-  // lambda_0 = res_0, lambda_1 = res_1 + 2 lambda_0,
-  // lambda_i = res_i + 2 lambda_{i-1} - lambda_{i-2}, carried in __float128:
-  // the recurrence has a double root at 1, so a rounding error made at row k
-  // grows like (i - k) by row i, beyond long double's reach once the
-  // multipliers reach ~1e6 (the reference offers the same quad-precision
-  // recurrence, qp_interior_point.cpp:1001-1029).
-  {
-    __float128 a = static_cast<__float128>(res[0]), b = 0;
-    lam[0] = res[0];
-    if (m > 1) {
-      b = static_cast<__float128>(res[1]) + 2 * a;
-      lam[1] = static_cast<ld>(b);
-    }
-    for (std::size_t i = 2; i < m; ++i) {
-      const __float128 c = static_cast<__float128>(res[i]) + 2 * b - a;
-      lam[i] = static_cast<ld>(c);
-      a = b;
-      b = c;
-    }
-  }
-  h_apply_a(x, slack);
-  h_apply_at(lam, N, atr);
-  Certificate c;
-  c.min_lambda = std::numeric_limits<ld>::infinity();
-  for (std::size_t j = 0; j < N; ++j) c.stationarity = std::max(c.stationarity, std::abs(res[j] - atr[j]));
-  for (std::size_t r = 0; r < m; ++r) {
-    c.primal = std::max(c.primal, std::max(0.0L, -slack[r]));
-    c.complementarity = std::max(c.complementarity, std::abs(lam[r] * slack[r]));
-    c.min_lambda = std::min(c.min_lambda, lam[r]);
-  }
-  if (m > 1)
-    c.consistency =
-        std::max(std::abs(res[N - 2] - (lam[m - 2] - 2.0L * lam[m - 1])), std::abs(res[N - 1] - lam[m - 1]));
-  ld scale = 1.0L;
-  for (std::size_t j = 0; j < N; ++j) scale = std::max({scale, std::abs(t[j]), std::abs(atr[j]), std::abs(x[j])});
-  const ld tol = 1e-8L * scale;
-  c.ok = std::isfinite(c.stationarity) && std::isfinite(c.complementarity) && c.primal <= tol &&
-         c.min_lambda >= -tol && c.stationarity <= tol && c.consistency <= tol && c.complementarity <= tol;
-  return c;
-}
-
-// Least-squares piecewise-linear fit through the knots of the inactive rows:
-// hat-basis normal equations (tridiagonal), Cholesky, interpolation.
-bool knot_fit(const std::vector<char>& active, const std::vector<ld>& t, std::vector<ld>& x) {
-  const int n = static_cast<int>(t.size()), m = n - 2;
-  std::vector<int> kn{0};
-  for (int r = 0; r < m; ++r)
-    if (!active[r]) kn.push_back(r + 1);
-  kn.push_back(n - 1);
-  kn.erase(std::unique(kn.begin(), kn.end()), kn.end());
-  if (kn.size() < 2) return false;
-  const std::size_t k = kn.size();
-  std::vector<ld> dg(k, 0.0L), off(k - 1, 0.0L), b(k, 0.0L);
-  for (std::size_t g = 0; g + 1 < k; ++g) {
-    const int a = kn[g], e = kn[g + 1], len = e - a;
-    if (len <= 0) return false;
-    const int last = (g + 2 == k) ? e : e - 1;
-    for (int j = a; j <= last; ++j) {
-      const ld u = static_cast<ld>(j - a) / static_cast<ld>(len);
-      dg[g] += (1.0L - u) * (1.0L - u);
-      b[g] += (1.0L - u) * t[j];
-      if (u != 0.0L) {
-        dg[g + 1] += u * u;
-        b[g + 1] += u * t[j];
-        off[g] += (1.0L - u) * u;
-      }
-    }
-  }
-  std::vector<ld> L(k), sub(k - 1);
-  for (std::size_t i = 0; i < k; ++i) {
-    ld p = dg[i];
-    if (i > 0) {
-      sub[i - 1] = off[i - 1] / L[i - 1];
-      p -= sub[i - 1] * sub[i - 1];
-    }
-    if (!(p > 0.0L) || !std::isfinite(p)) return false;
-    L[i] = std::sqrt(p);
-  }
-  for (std::size_t i = 0; i < k; ++i) {
-    if (i > 0) b[i] -= sub[i - 1] * b[i - 1];
-    b[i] /= L[i];
-  }
-  for (std::size_t i = k; i-- > 0;) {
-    if (i + 1 < k) b[i] -= sub[i] * b[i + 1];
-    b[i] /= L[i];
-  }
-  x.assign(n, 0.0L);
-  for (std::size_t g = 0; g + 1 < k; ++g) {
-    const int a = kn[g], e = kn[g + 1], len = e - a;
-    const int last = (g + 2 == k) ? e : e - 1;
-    for (int j = a; j <= last; ++j) {
-      const ld u = static_cast<ld>(j - a) / static_cast<ld>(len);
-      x[j] = (1.0L - u) * b[g] + u * b[g + 1];
-    }
-  }
-  return true;
-}
-
-// Active rows start as s < z; each pass fits on the knots, certifies, and
-// flips the rows the certificate blames, at most min(1024, max(8, m/10)) by
-// evidence. Returns passes used, or -1.
-int polish(const std::vector<ld>& t, const std::vector<ld>& s, const std::vector<ld>& z, std::vector<ld>& x,
-           Certificate& cert) {
-  const int m = static_cast<int>(s.size());
-  std::vector<char> act(m);
-  for (int r = 0; r < m; ++r) act[r] = s[r] < z[r];
-  ld sc0 = 1.0L;
-  for (ld v : t) sc0 = std::max(sc0, std::abs(v));
-  for (int pass = 0; pass < 50; ++pass) {
-    std::vector<ld> cand, lam, cs;
-    if (!knot_fit(act, t, cand)) return -1;
-    cert = certify(t, cand, lam, cs);
-    ld sc = sc0;
-    for (ld v : cand) sc = std::max(sc, std::abs(v));
-    const ld tol = 1e-8L * sc;
-    if (cert.ok) {
-      x.swap(cand);
-      return pass + 1;
-    }
-    std::vector<char> nx = act;
-    std::vector<std::pair<ld, int>> ev;
-    int changes = 0;
-    for (int r = 0; r < m; ++r) {
-      if (act[r] && lam[r] < -tol) {
-        nx[r] = 0;
-        ev.emplace_back(std::abs(lam[r]), r);
-      } else if (!act[r] && (cs[r] < -tol || (lam[r] > tol && lam[r] * cs[r] > tol))) {
-        nx[r] = 1;
-        ev.emplace_back(std::max(-cs[r], lam[r] * cs[r]), r);
-      }
-      if (nx[r] != act[r]) ++changes;
-    }
-    const int cap = std::min(1024, std::max(8, m / 10));
-    if (changes > cap) {
-      std::sort(ev.begin(), ev.end(), [](const auto& a, const auto& b) { return a.first > b.first; });
-      nx = act;
-      changes = 0;
-      for (int q = 0; q < cap && q < static_cast<int>(ev.size()); ++q) {
-        nx[ev[q].second] = act[ev[q].second] ? 0 : 1;
-        ++changes;
-      }
-    }
-    if (changes == 0) return -1;
-    act.swap(nx);
-  }
-  return -1;
-}
+// ---- host -------------------------------------------------------------------
 
 int levels_for(int n, Levels* L) {
   int N = (n + 1) / 2;
@@ -804,6 +625,9 @@ int levels_for(int n, Levels* L) {
 }
 
 }  // namespace
+
+using sovqp::Certificate;
+using sovqp::ld;
 
 extern "C" {
 
@@ -914,20 +738,20 @@ int sov_qp_liswet_solve(int32_t n, const double* target, const sov_qp_options* o
     }
   }
   // knot-space polish and certificate on the host
-  std::vector<ld> t(target, target + n), x(xh.begin(), xh.end()), sv(sh.begin(), sh.end()), zv(zh.begin(), zh.end());
+  std::vector<ld> t(target, target + n), x(xh.begin(), xh.end());
   Certificate cert;
   std::vector<ld> lam, slack;
-  int passes = 0;
   if (o.polish) {
-    std::vector<ld> xp = x;
-    passes = polish(t, sv, zv, xp, cert);
-    if (passes > 0) x.swap(xp);
+    std::vector<char> guess(m);
+    for (int r = 0; r < m; ++r) guess[r] = sh[r] < zh[r];
+    std::vector<ld> xp;
+    sovqp::PolishStats st;
+    if (sovqp::active_set_polish(t, guess, xp, cert, st)) x.swap(xp);
+    res->polish_fits = st.fits;
   }
-  if (passes <= 0) cert = certify(t, x, lam, slack);
-  else certify(t, x, lam, slack);
+  cert = sovqp::certify(t, x, lam, slack);
   ld obj = 0.0L;
   for (int j = 0; j < n; ++j) obj += 0.5L * x[j] * x[j] - t[j] * x[j];
-  res->polish_passes = passes;
   res->objective = static_cast<double>(obj);
   res->primal = static_cast<double>(cert.primal);
   res->stationarity = static_cast<double>(cert.stationarity);
@@ -941,6 +765,37 @@ int sov_qp_liswet_solve(int32_t n, const double* target, const sov_qp_options* o
     for (int r = 0; r < m; ++r) lambda_out[r] = static_cast<double>(lam[r]);
   res->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   sov::count_operation();
+  return SOV_OK;
+}
+
+int sov_qp_liswet_polish(int32_t n, const double* target, const double* s, const double* z, sov_qp_result* res,
+                         double* x_out, double* lambda_out) {
+  if (n < 3 || !target || !s || !z || !res) return SOV_ERR_ARG;
+  *res = sov_qp_result{};
+  const auto t0 = std::chrono::steady_clock::now();
+  const int m = n - 2;
+  std::vector<ld> t(target, target + n), x, lam, slack;
+  std::vector<char> guess(m);
+  for (int r = 0; r < m; ++r) guess[r] = s[r] < z[r];
+  Certificate cert;
+  sovqp::PolishStats st;
+  if (!sovqp::active_set_polish(t, guess, x, cert, st)) x.assign(t.begin(), t.end());
+  cert = sovqp::certify(t, x, lam, slack);
+  ld obj = 0.0L;
+  for (int j = 0; j < n; ++j) obj += 0.5L * x[j] * x[j] - t[j] * x[j];
+  res->status = cert.ok ? 0 : 6;
+  res->polish_fits = st.fits;
+  res->objective = static_cast<double>(obj);
+  res->primal = static_cast<double>(cert.primal);
+  res->stationarity = static_cast<double>(cert.stationarity);
+  res->complementarity = static_cast<double>(cert.complementarity);
+  res->min_lambda = static_cast<double>(cert.min_lambda);
+  res->consistency = static_cast<double>(cert.consistency);
+  if (x_out)
+    for (int j = 0; j < n; ++j) x_out[j] = static_cast<double>(x[j]);
+  if (lambda_out)
+    for (int r = 0; r < m; ++r) lambda_out[r] = static_cast<double>(lam[r]);
+  res->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   return SOV_OK;
 }
 

@@ -130,6 +130,25 @@ def solve_liswet(target, options: QpInteriorPointOptions | None = None) -> dict:
     return out
 
 
+def polish_liswet(target, s, z) -> dict:
+    """The host polish alone from an interior point (s, z) of the QP for
+    `target` (sov_qp_liswet_polish; needs no device): dict with status (0
+    certified), objective, x, lam and the certificate."""
+    lib = K.load()
+    t = K.f64(target)
+    n = len(t)
+    if n < 3 or len(s) != n - 2 or len(z) != n - 2:
+        raise ValueError("target needs n >= 3 entries and s, z n - 2 each")
+    x = np.zeros(n)
+    lam = np.zeros(n - 2)
+    res = K.QpResult()
+    K.check(lib.sov_qp_liswet_polish(n, K.pf(t), K.pf(K.f64(s)), K.pf(K.f64(z)), C.byref(res), K.pf(x), K.pf(lam)),
+            "sov_qp_liswet_polish")
+    out = res.as_dict()
+    out.update(x=x, lam=lam)
+    return out
+
+
 class QpInteriorPointSolver:
     """QpInteriorPointSolver (qp_interior_point.hpp:24-31) for the structured
     shape. solve(QpModel) -> dict (status 0 Optimal with a certificate, 6

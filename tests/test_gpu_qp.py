@@ -35,7 +35,7 @@ def run(target, precision="mixed", **kw):
 
 
 def summary(r):
-    keys = ("status", "ipm_status", "iterations", "polish_passes", "fp32_factorizations", "fp64_factorizations",
+    keys = ("status", "ipm_status", "iterations", "polish_fits", "fp32_factorizations", "fp64_factorizations",
             "refinements", "objective", "stationarity", "complementarity", "min_lambda", "ipm_primal", "ipm_dual",
             "ipm_complementarity", "kernel_ms")
     return {k: r[k] for k in keys}

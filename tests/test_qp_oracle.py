@@ -13,11 +13,12 @@
   projects onto its least-squares line (all rows active).
 - The certificate rejects planted mistakes (a perturbed or mirrored x).
 Also the structure detector's acceptance rules (detect_liswet_structured,
-solver/qp/src/qp_interior_point.cpp:892-967) on model variants.
+solver/qp/src/qp_interior_point.cpp:892-967) on model variants, and the
+library's host polish (sov_qp_liswet_polish, which needs no device) from
+good, empty, full and perturbed active-set guesses against the same pins.
 """
 from __future__ import annotations
 
-import itertools
 import json
 import os
 
@@ -181,3 +182,63 @@ def test_synthetic_target_shape():
     resid = t - (np.sin(3 * u) + u * u)
     assert abs(resid.std() - 0.01) < 5e-4 and abs(resid.mean()) < 5e-4
     np.testing.assert_array_equal(t, qp.liswet_target(20002))
+
+
+# ---- the library's host polish (sov_qp_liswet_polish; no device needed) ----
+
+def _guess_arrays(active):
+    """(s, z) whose guess {s < z} is `active`."""
+    active = np.asarray(active, dtype=bool)
+    return np.where(active, 0.5, 2.0), np.where(active, 2.0, 0.5)
+
+
+def _true_active(x):
+    a = x[:-2] - 2.0 * x[1:-1] + x[2:]
+    return np.abs(a) <= 1e-12 * max(1.0, np.max(np.abs(x)))
+
+
+@pytest.mark.parametrize("n", [200, 1000, 10000, 20000])
+@pytest.mark.parametrize("guess", ["all_active", "none_active", "perturbed"])
+def test_library_polish_recovers_reference_solution(n, guess):
+    target, xref = FIX[f"target_{n}"], FIX[f"x_{n}"]
+    truth = _true_active(xref)
+    if guess == "all_active":
+        act = np.ones(n, dtype=bool)
+    elif guess == "none_active":
+        act = np.zeros(n, dtype=bool)
+    else:
+        rng = np.random.default_rng(n)
+        act = truth.copy()
+        flip = rng.choice(n, size=max(3, n // 50), replace=False)
+        act[flip] = ~act[flip]
+    s, z = _guess_arrays(act)
+    r = qp.polish_liswet(target, s, z)
+    assert r["status"] == 0, {k: r[k] for k in ("polish_fits", "stationarity", "min_lambda", "primal")}
+    gold = float(GOLD[str(n)]["objective"])
+    assert abs(r["objective"] - gold) <= 1e-12 * abs(gold)
+    assert np.max(np.abs(r["x"] - xref)) <= 1e-9
+    cert, lam = O.liswet_certify(target, r["x"])
+    assert cert["ok"]
+    np.testing.assert_allclose(r["lam"], lam, rtol=1e-9, atol=1e-9 * np.max(np.abs(lam)))
+
+
+def test_library_polish_matches_brute_force_from_any_guess():
+    rng = np.random.default_rng(7)
+    for c in small_cases(30, seed=99):
+        ref = brute_force(c)
+        act = rng.random(len(c) - 2) < 0.5
+        r = qp.polish_liswet(c, *_guess_arrays(act))
+        assert r["status"] == 0
+        assert np.max(np.abs(r["x"] - ref)) <= 1e-8 * max(1.0, np.max(np.abs(ref)))
+
+
+def test_library_polish_closed_forms():
+    t = np.linspace(0.0, 1.0, 400)
+    convex = 1.0 + 0.5 * t + 3.0 * t * t
+    r = qp.polish_liswet(convex, *_guess_arrays(np.ones(398, dtype=bool)))
+    assert r["status"] == 0
+    np.testing.assert_allclose(r["x"], convex, atol=1e-9)
+    concave = np.sin(3.0 * t)
+    r = qp.polish_liswet(concave, *_guess_arrays(np.zeros(398, dtype=bool)))
+    assert r["status"] == 0
+    np.testing.assert_allclose(r["x"], np.polyval(np.polyfit(t, concave, 1), t), atol=1e-9)
