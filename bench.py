@@ -33,6 +33,7 @@ n=20000 instance), with the oracle's one-thread solve as its CPU baseline.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
                        [--paths mixed,fp64] [--no-branching] [--no-qp]
+       python bench.py --only-qp [--steps K] [--no-cpu]   (the QP leg alone)
        python bench.py --copy-probe   (a plain 256 MiB device copy, three
                                        times: the calibration run for the
                                        FETCH_SIZE / WRITE_SIZE counters)
@@ -260,6 +261,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-branching", action="store_true")
     ap.add_argument("--no-qp", action="store_true")
+    ap.add_argument("--only-qp", action="store_true", help="run the separable QP leg alone (profiling)")
     ap.add_argument("--paths", default="mixed,fp64")
     ap.add_argument("--m", type=int, default=8192)
     ap.add_argument("--n", type=int, default=32768)
@@ -281,6 +283,12 @@ def main():
               file=sys.stderr)
         sys.exit(2)
     paths = [p for p in args.paths.split(",") if p]
+    if args.only_qp:
+        from sovereign_amd import _capi
+
+        _capi.load()
+        print(json.dumps({"separable_qp": qp_leg(max(1, args.steps), not args.no_cpu)}), flush=True)
+        return
 
     from sovereign_amd import _capi, api, problems as P
 
