@@ -86,6 +86,111 @@ __device__ __forceinline__ float bcast_from<float>(float v, int src) {
   return __int_as_float(__builtin_amdgcn_ds_bpermute(4 * src, __float_as_int(v)));
 }
 
+constexpr int kLeaf = 16;             // leaf width of the diagonal sub-block factorization
+constexpr int kLeafLd = kLeaf + 1;
+
+// Orders the LDS accesses of one wavefront (no workgroup barrier).
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+}
+
+// Cholesky factor L and inverse X = L^-1 of the h x h (h <= 16) block at D
+// (column-major, leading dimension kPanelLd) by one wavefront, in registers.
+// The 64 lanes form an 8 x 8 grid (p = lane & 7, q = lane >> 3); lane (p, q)
+// holds the elements (p + 8r, q + 8s), r, s < 2, of the block (f) and of X
+// (x). Step j scales column j of L and row j of X, then applies the rank-1
+// updates f -= L(:, j) L(:, j)^T below and right of j and x -= L(:, j) X(j, :)
+// below j; the factors of the update are fetched from the lanes that own
+// them and zeroed where the update does not apply, so the updates themselves
+// are unconditional. Writes L (lower), X (strict upper: X(i, k) at
+// D[i * kPanelLd + k]) and 1 / L(j, j) to xd[j]. thr (lane j): drop
+// threshold of column j. Pivot rule as described above the kernel.
+template <typename T>
+__device__ __forceinline__ void leaf_factor(T* D, int h, double thr, T* xd, FactorFlags* flags, int lane) {
+  constexpr int NR = kLeaf / 8;
+  const int p = lane & 7, q = lane >> 3;
+  T f[NR][NR], x[NR][NR];
+#pragma unroll
+  for (int r = 0; r < NR; ++r)
+#pragma unroll
+    for (int s2 = 0; s2 < NR; ++s2) {
+      const int i = p + 8 * r, k = q + 8 * s2;
+      f[r][s2] = (i < h && k <= i) ? D[k * kPanelLd + i] : T(0);
+      x[r][s2] = i == k ? T(1) : T(0);
+    }
+  // The pivot of step j + 1 is formed as soon as column j is scaled,
+  // a(j+1, j+1) - L(j+1, j)^2 from two lane reads, so that its reciprocal
+  // square root overlaps the broadcasts and updates of step j.
+  double dnext = static_cast<double>(read_lane(f[0][0], 0));
+#pragma unroll
+  for (int j = 0; j < kLeaf; ++j) {
+    if (j < h) {
+      const int jp = j & 7, js = j >> 3;
+      const double dj = dnext;
+      const bool finite = isfinite(dj);
+      const bool drop = finite && dj <= read_lane(thr, j);
+      const bool bad = !drop && (!finite || dj < 1e-30);
+      // 1/sqrt(dj): hardware estimate and two Newton steps; sqrt = dj / sqrt(dj)
+      double rs = __builtin_amdgcn_rsq(dj);
+      rs = rs * (1.5 - 0.5 * dj * rs * rs);
+      rs = rs * (1.5 - 0.5 * dj * rs * rs);
+      const T piv = drop ? DropPivot<T>::value : bad ? T(1) : static_cast<T>(dj * rs);
+      const T rpiv = drop ? T(1) / DropPivot<T>::value : bad ? T(1) : static_cast<T>(rs);
+      if (lane == 0) {
+        if (drop) atomicAdd(&flags->replaced, 1);
+        if (bad) atomicOr(&flags->status, 1);
+        xd[j] = rpiv;
+      }
+      if (q == jp) {  // column j of L: the pivot, and l = a / pivot below it
+#pragma unroll
+        for (int r = 0; r < NR; ++r) {
+          const int i = p + 8 * r;
+          f[r][js] = i == j ? piv : (i > j ? f[r][js] * rpiv : f[r][js]);
+        }
+      }
+      if (j + 1 < h) {
+        const int j1 = j + 1, j1p = j1 & 7, j1s = j1 >> 3;
+        const T l = read_lane(f[j1s][js], j1p + 8 * jp);      // L(j+1, j)
+        const T a = read_lane(f[j1s][j1s], 9 * j1p);          // a(j+1, j+1) before this step's update
+        dnext = static_cast<double>(a) - static_cast<double>(l) * static_cast<double>(l);
+      }
+      if (p == jp) {  // row j of X
+#pragma unroll
+        for (int s2 = 0; s2 < NR; ++s2) x[js][s2] *= rpiv;
+      }
+      T lr[NR], lc[NR], xr[NR];
+#pragma unroll
+      for (int r = 0; r < NR; ++r) {
+        const T v = bcast_from(f[r][js], p + 8 * jp);  // L(p + 8r, j)
+        lr[r] = p + 8 * r > j ? v : T(0);
+      }
+#pragma unroll
+      for (int s2 = 0; s2 < NR; ++s2) {
+        const T v = bcast_from(f[s2][js], q + 8 * jp);  // L(q + 8 s2, j)
+        lc[s2] = q + 8 * s2 > j ? v : T(0);
+        xr[s2] = bcast_from(x[js][s2], jp + 8 * q);     // X(j, q + 8 s2)
+      }
+#pragma unroll
+      for (int r = 0; r < NR; ++r)
+#pragma unroll
+        for (int s2 = 0; s2 < NR; ++s2) {
+          f[r][s2] -= lr[r] * lc[s2];
+          x[r][s2] -= lr[r] * xr[s2];
+        }
+    }
+  }
+#pragma unroll
+  for (int r = 0; r < NR; ++r)
+#pragma unroll
+    for (int s2 = 0; s2 < NR; ++s2) {
+      const int i = p + 8 * r, k = q + 8 * s2;
+      if (i < h && k <= i) D[k * kPanelLd + i] = f[r][s2];  // L(i, k)
+      if (i < h && k < i) D[i * kPanelLd + k] = x[r][s2];   // X(i, k), above the diagonal
+    }
+}
+
 template <typename T>
 __global__ __launch_bounds__(kPanelThreads) void potrf_panel_kernel(T* __restrict__ a, long long lda, int k0, int nb,
                                                                     const double* __restrict__ diag0,
@@ -119,80 +224,70 @@ __global__ __launch_bounds__(kPanelThreads) void potrf_panel_kernel(T* __restric
     const int rb = nb - c - sb;        // rows below it (sb == kSub when rb > 0)
     T* D = S + c * (kPanelLd + 1);     // its diagonal block: element (i, k) at D[k * kPanelLd + i]
 
-    // ---- A: diagonal block and its inverse in the registers of wavefront 0.
-    // The 64 lanes form an 8 x 8 grid (p = lane & 7, q = lane >> 3); lane
-    // (p, q) holds the elements (p + 8r, q + 8s), r, s < 4, of the block (f)
-    // and of its inverse X (x). Step j scales column j of L and row j of X,
-    // then applies the rank-1 updates; each lane fetches the four L(i, j) of
-    // its rows, the four L(k, j) and X(j, k) of its columns from the lanes
-    // that own them.
+    // ---- A: the sb x sb diagonal block and its inverse, by wavefront 0
+    // alone (no workgroup barrier): two 16 x 16 leaves factored in
+    // registers (leaf_factor) and joined by three 16 x 16 x 16 MFMA products
+    // from LDS: L21 = A21 X11^T, A22 -= L21 L21^T, X21 = -X22 (L21 X11).
     if (wave == 0) {
-      const int p = lane & 7, q = lane >> 3;
-      T f[4][4], x[4][4];
+      const int h1 = min(kLeaf, sb), h2 = sb - h1;
+      leaf_factor<T>(D, h1, lane < h1 ? 1e-13 * fabs(d0[c + lane]) : 0.0, xdiag + c, flags, lane);
+      if (h2 > 0) {
+        wave_lds_sync();
+        T* Ys = Yt;  // 16 x 16 scratch, Y(i, k) at Ys[i * kLeafLd + k]
+        {  // L21 = A21 X11^T over rows kLeaf.., columns 0..kLeaf-1
+          acc_t acc = acc_t{0, 0, 0, 0};
 #pragma unroll
-      for (int r = 0; r < 4; ++r)
+          for (int k = 0; k < kLeaf; k += 4) {
+            const int kk = k + lk;
+            const T av = li < h2 ? D[kk * kPanelLd + kLeaf + li] : T(0);
+            const T bv = li > kk ? D[li * kPanelLd + kk] : (li == kk ? xdiag[c + li] : T(0));
+            acc = Mfma<T>::step(av, bv, acc);
+          }
 #pragma unroll
-        for (int s2 = 0; s2 < 4; ++s2) {
-          const int i = p + 8 * r, k = q + 8 * s2;
-          f[r][s2] = (i < sb && k <= i) ? D[k * kPanelLd + i] : T(0);
-          x[r][s2] = i == k ? T(1) : T(0);
+          for (int g = 0; g < 4; ++g) {
+            const int i = Mfma<T>::row(lk, g);
+            if (i < h2) D[li * kPanelLd + kLeaf + i] = acc[g];
+          }
         }
-      const double thr = lane < sb ? 1e-13 * fabs(d0[c + lane]) : 0.0;  // drop threshold of column `lane`
+        wave_lds_sync();
+        {  // A22 -= L21 L21^T (lower part) and Y = L21 X11
+          acc_t acc = acc_t{0, 0, 0, 0}, yacc = acc_t{0, 0, 0, 0};
 #pragma unroll
-      for (int j = 0; j < kSub; ++j) {
-        if (j < sb) {
-          const int jp = j & 7, js = j >> 3;
-          const double dj = static_cast<double>(read_lane(f[js][js], 9 * jp));
-          const bool finite = isfinite(dj);
-          const bool drop = finite && dj <= read_lane(thr, j);
-          const bool bad = !drop && (!finite || dj < 1e-30);
-          // 1/sqrt(dj): hardware estimate and two Newton steps; sqrt = dj / sqrt(dj)
-          double rs = __builtin_amdgcn_rsq(dj);
-          rs = rs * (1.5 - 0.5 * dj * rs * rs);
-          rs = rs * (1.5 - 0.5 * dj * rs * rs);
-          const T piv = drop ? DropPivot<T>::value : bad ? T(1) : static_cast<T>(dj * rs);
-          const T rpiv = drop ? T(1) / DropPivot<T>::value : bad ? T(1) : static_cast<T>(rs);
-          if (lane == 0) {
-            if (drop) atomicAdd(&flags->replaced, 1);
-            if (bad) atomicOr(&flags->status, 1);
-            xdiag[c + j] = rpiv;
-          }
-          if (q == jp) {  // column j of L: the pivot, and l = a / pivot below it
-#pragma unroll
-            for (int r = 0; r < 4; ++r) {
-              const int i = p + 8 * r;
-              f[r][js] = i == j ? piv : (i > j ? f[r][js] * rpiv : f[r][js]);
-            }
-          }
-          if (p == jp) {  // row j of X
-#pragma unroll
-            for (int s2 = 0; s2 < 4; ++s2) x[js][s2] *= rpiv;
-          }
-          T lr[4], lc[4], xr[4];
-#pragma unroll
-          for (int r = 0; r < 4; ++r) lr[r] = bcast_from(f[r][js], p + 8 * jp);   // L(p + 8r, j)
-#pragma unroll
-          for (int s2 = 0; s2 < 4; ++s2) {
-            lc[s2] = bcast_from(f[s2][js], q + 8 * jp);  // L(q + 8 s2, j)
-            xr[s2] = bcast_from(x[js][s2], jp + 8 * q);  // X(j, q + 8 s2)
+          for (int k = 0; k < kLeaf; k += 4) {
+            const int kk = k + lk;
+            const T lv = li < h2 ? D[kk * kPanelLd + kLeaf + li] : T(0);  // L21(li, kk)
+            acc = Mfma<T>::step(lv, lv, acc);
+            const T xv = kk > li ? D[kk * kPanelLd + li] : (kk == li ? xdiag[c + li] : T(0));  // X11(kk, li)
+            yacc = Mfma<T>::step(lv, xv, yacc);
           }
 #pragma unroll
-          for (int r = 0; r < 4; ++r)
+          for (int g = 0; g < 4; ++g) {
+            const int i = Mfma<T>::row(lk, g);
+            if (i < h2 && i >= li) D[(kLeaf + li) * kPanelLd + kLeaf + i] -= acc[g];
+            Ys[i * kLeafLd + li] = yacc[g];
+          }
+        }
+        wave_lds_sync();
+        T* D2 = D + kLeaf * (kPanelLd + 1);
+        leaf_factor<T>(D2, h2, lane < h2 ? 1e-13 * fabs(d0[c + kLeaf + lane]) : 0.0, xdiag + c + kLeaf, flags,
+                       lane);
+        wave_lds_sync();
+        {  // X21 = -X22 Y
+          acc_t acc = acc_t{0, 0, 0, 0};
 #pragma unroll
-            for (int s2 = 0; s2 < 4; ++s2) {
-              if (q + 8 * s2 > j) f[r][s2] -= lr[r] * lc[s2];
-              if (p + 8 * r > j) x[r][s2] -= lr[r] * xr[s2];
-            }
+          for (int k = 0; k < kLeaf; k += 4) {
+            const int kk = k + lk;
+            const T av = li > kk ? D2[li * kPanelLd + kk] : (li == kk ? xdiag[c + kLeaf + li] : T(0));
+            const T bv = Ys[kk * kLeafLd + li];
+            acc = Mfma<T>::step(av, bv, acc);
+          }
+#pragma unroll
+          for (int g = 0; g < 4; ++g) {
+            const int i = Mfma<T>::row(lk, g);
+            if (i < h2) D[(kLeaf + i) * kPanelLd + li] = -acc[g];
+          }
         }
       }
-#pragma unroll
-      for (int r = 0; r < 4; ++r)
-#pragma unroll
-        for (int s2 = 0; s2 < 4; ++s2) {
-          const int i = p + 8 * r, k = q + 8 * s2;
-          if (i < sb && k <= i) D[k * kPanelLd + i] = f[r][s2];  // L(i, k)
-          if (i < sb && k < i) D[i * kPanelLd + k] = x[r][s2];   // X(i, k), above the diagonal
-        }
     }
     __syncthreads();
 
