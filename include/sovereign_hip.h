@@ -147,7 +147,8 @@ typedef struct {
   int32_t precision;
   double mixed_switch_mu; /* 1e-6 */
   int32_t refine_max;     /* 30: PCG iterations per solve at most */
-  double refine_tol;      /* 1e-12: relative residual target of a solve */
+  double refine_tol;      /* 1e-12: relative residual target of a solve (an FP32-preconditioned
+                           * solve stops at max(refine_tol, min(1e-6, 1e-3 mu))) */
 } sov_ipm_options;
 
 typedef struct {

@@ -389,11 +389,16 @@ int ipm_run(sov_ipm* h, const double* b_h, const double* c_h, const sov_ipm_opti
   };
   // Normal-equation solve with the precision of the current factor; a
   // mixed-precision solve that cannot reach its tolerance refactors in FP64.
+  // The preconditioned CG of an FP32 factor stops at the relative residual
+  // max(refine_tol, min(1e-6, 1e-3 mu)): an inexact Newton step while mu is
+  // large, exact to refine_tol as mu falls (the factorizations turn FP64 at
+  // mixed_switch_mu, where the solves are direct).
   int cur_prec = SOV_PREC_FP64;
+  double pcg_tol = solve_tol;
   auto newton_solve = [&](double* v) -> int {
     if (cur_prec == SOV_PREC_FP32) {
       SOV_HIP_TRY(hipMemcpyAsync(rhs_save.p, v, m * sizeof(double), hipMemcpyDeviceToDevice, st));
-      int r = ne.solve(v, st, solve_tol, solve_max);
+      int r = ne.solve(v, st, pcg_tol, solve_max);
       if (r < 0) return r;
       res->refine_iterations += ne.last_pcg_iterations();
       if (r == SOV_OK) {
@@ -516,6 +521,7 @@ int ipm_run(sov_ipm* h, const double* b_h, const double* c_h, const sov_ipm_opti
 
     // Factorization of A D A^T, D = x / s (d was written by resid_cols).
     cur_prec = (mixed && !fp64_locked && S.mu >= opt.mixed_switch_mu) ? SOV_PREC_FP32 : SOV_PREC_FP64;
+    pcg_tol = std::max(solve_tol, std::min(1e-6, 1e-3 * S.mu));
     (void)hipEventRecord(ev.e[0], st);
     if ((rc = ne.factor(d.p, cur_prec, st)) != SOV_OK) return rc;
     (void)hipEventRecord(ev.e[1], st);

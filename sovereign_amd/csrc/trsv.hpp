@@ -13,43 +13,48 @@
 
 // Expects to be included inside namespace sov (anonymous namespace) of chol.hip.
 
-constexpr int kGemvRows = 64;  // rows of y per workgroup (one per lane)
+constexpr int kGemvRows = 64;    // rows of y per workgroup (one per lane)
+constexpr int kGemvWaves = 16;   // wavefronts per workgroup, splitting the columns
 
 // y[r] = s (SUB: y[r] -= s), s = sum_{k < K} A(r, k) x[k], r < M; A column-
 // major. TRI: A is lower triangular (columns k > r are skipped). K <= kSuper.
-// A workgroup owns 64 rows; its four wavefronts take every fourth column and
-// their partial sums are added in a fixed order (deterministic).
+// A workgroup owns 64 rows; its 16 wavefronts take every 16th column (many
+// loads in flight per CU: the products are latency-bound at these sizes)
+// and their partial sums are added in a fixed order (deterministic).
 template <typename T, bool SUB, bool TRI>
-__global__ __launch_bounds__(256) void gemv_n_kernel(int M, int K, const T* __restrict__ A, long long lda,
-                                                     const double* __restrict__ x, double* __restrict__ y) {
+__global__ __launch_bounds__(64 * kGemvWaves) void gemv_n_kernel(int M, int K, const T* __restrict__ A,
+                                                                 long long lda, const double* __restrict__ x,
+                                                                 double* __restrict__ y) {
   __shared__ double xs[kSuper];
-  __shared__ double part[4][kGemvRows];
+  __shared__ double part[kGemvWaves][kGemvRows];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int r0 = blockIdx.x * kGemvRows;
   const int kend = TRI ? min(K, r0 + kGemvRows) : K;
-  for (int k = threadIdx.x; k < kend; k += 256) xs[k] = x[k];
+  for (int k = threadIdx.x; k < kend; k += 64 * kGemvWaves) xs[k] = x[k];
   __syncthreads();
   const int r = r0 + lane;
   double s0 = 0.0, s1 = 0.0;
   if (r < M) {
     const T* col = A + r;
+    constexpr int W = kGemvWaves;
     int k = w;
-    for (; k + 28 < kend; k += 32) {
-      T v[8];
+    for (; k + 3 * W < kend; k += 4 * W) {
+      T v[4];
 #pragma unroll
-      for (int q = 0; q < 8; ++q) v[q] = col[static_cast<long long>(k + 4 * q) * lda];
-#pragma unroll
-      for (int q = 0; q < 8; q += 2) {
-        s0 = fma(static_cast<double>(v[q]), xs[k + 4 * q], s0);
-        s1 = fma(static_cast<double>(v[q + 1]), xs[k + 4 * q + 4], s1);
-      }
+      for (int q = 0; q < 4; ++q) v[q] = col[static_cast<long long>(k + W * q) * lda];
+      s0 = fma(static_cast<double>(v[0]), xs[k], s0);
+      s1 = fma(static_cast<double>(v[1]), xs[k + W], s1);
+      s0 = fma(static_cast<double>(v[2]), xs[k + 2 * W], s0);
+      s1 = fma(static_cast<double>(v[3]), xs[k + 3 * W], s1);
     }
-    for (; k < kend; k += 4) s0 = fma(static_cast<double>(col[static_cast<long long>(k) * lda]), xs[k], s0);
+    for (; k < kend; k += W) s0 = fma(static_cast<double>(col[static_cast<long long>(k) * lda]), xs[k], s0);
   }
   part[w][lane] = s0 + s1;
   __syncthreads();
   if (w == 0 && r < M) {
-    const double t = (part[0][lane] + part[1][lane]) + (part[2][lane] + part[3][lane]);
+    double t = 0.0;
+#pragma unroll
+    for (int q = 0; q < kGemvWaves; ++q) t += part[q][lane];
     y[r] = SUB ? y[r] - t : t;
   }
 }
