@@ -62,9 +62,11 @@ int sov_sync(void);
 /* The library's HIP stream (hipStream_t), for callers that time kernels. */
 void* sov_stream(void);
 
-/* Kernel timing: with timing on, the factorization and solve kernels are
+/* Kernel timing: with timing on (1), the factorization and solve kernels are
  * bracketed by HIP events on the streams they run on (recording does not
- * wait; the totals are collected after one stream synchronisation).
+ * wait; the totals are collected after one stream synchronisation); 2 times
+ * the bulk trailing-update SYRK only (slots 0-2 and 9-11), which keeps the
+ * event traffic off the critical path of the factorization.
  * sov_kernel_times fills out[0..11] with the totals since the last call and
  * resets them:
  *   0 FP64 trailing-update SYRK ms (the bulk update of each panel, MFMA),

@@ -25,9 +25,10 @@
 namespace sov {
 namespace {
 
-std::atomic<bool> g_timing{false};
+std::atomic<int> g_timing{0};  // 0 off, 1 every kernel class, 2 the bulk SYRK only
 
 constexpr int kPanelLd = kPanel + 1;  // LDS leading dimension of a panel block
+constexpr int kPanelBuffers = 3;      // L21 buffers in rotation (see factor)
 
 // ---------------------------------------------------------------------------
 // Diagonal block: Cholesky of the nb x nb block (nb <= kPanel = 128) and the
@@ -469,7 +470,8 @@ EventPool& pool() {
 
 // Opens a timed pair on `s` when timing is on (nullptr otherwise).
 TimedPair* timed_begin(hipStream_t s, int kind) {
-  if (!g_timing.load()) return nullptr;
+  const int level = g_timing.load();
+  if (level == 0 || (level == 2 && kind != kTimeSyrk && kind != kTimeSyrk32)) return nullptr;
   TimedPair* p = pool().next();
   if (!p) return nullptr;
   p->kind = kind;
@@ -486,8 +488,8 @@ void timed_end(TimedPair* p, hipStream_t s, int launches, double flops) {
 }
 
 
-void set_kernel_timing(bool on) { g_timing.store(on); }
-bool kernel_timing() { return g_timing.load(); }
+void set_kernel_timing(int level) { g_timing.store(level); }
+bool kernel_timing() { return g_timing.load() == 1; }
 KernelTimes& kernel_times() {
   static KernelTimes t;
   return t;
@@ -547,7 +549,7 @@ int DenseCholesky<T>::reserve(int m) {
   panel_ = sup_inv_ = tmp_ = nullptr;
   work_ = nullptr;
   flags_ = nullptr;
-  SOV_HIP_TRY(hipMalloc(&panel_, sizeof(T) * static_cast<std::size_t>(m) * kPanel * 2));
+  SOV_HIP_TRY(hipMalloc(&panel_, sizeof(T) * static_cast<std::size_t>(m) * kPanel * kPanelBuffers));
   const std::size_t nsb = static_cast<std::size_t>((m + kSuper - 1) / kSuper);
   const std::size_t sq = static_cast<std::size_t>(kSuper) * kSuper;
   // Blocks above the diagonal of each super-block stay zero: nothing writes them.
@@ -587,19 +589,17 @@ int DenseCholesky<T>::factor(T* a, int m, const double* diag0, hipStream_t s) {
     // the panel inverse is the diagonal block p % 8 of super-block p / 8
     T* pinv = sup_inv_ + static_cast<long long>(k0 / kSuper) * kSuper * kSuper +
               static_cast<long long>(k0 % kSuper) * (kSuper + 1);
-    T* L21 = panel_ + (p & 1) * pstride;
+    T* L21 = panel_ + (p % kPanelBuffers) * pstride;
     TimedPair* tp = timed_begin(s, kTimePanel);
     potrf_panel_kernel<T><<<1, kPanelThreads, panel_lds_bytes<T>(), s>>>(a, lda, k0, nb, diag0, flags_, pinv, kSuper);
     SOV_LAUNCH_CHECK();
     if (below > 0) {
-      // L21 = A21 inv(L11)^T on MFMA, out of place, then back into the factor.
+      // L21 = A21 inv(L11)^T on MFMA, out of place (the updates read it
+      // there; the copy back into the factor runs on the side stream).
       const T* a21 = a + (k0 + nb) + static_cast<long long>(k0) * lda;
       SOV_HIP_TRY((gemm_update<T, false, false, true>(below, nb, nb, a21, lda, pinv, kSuper, L21, lda, T(1), s)));
-      copy_panel_kernel<T><<<blocks_for(static_cast<long long>(below) * nb, 256), 256, 0, s>>>(L21, a, lda, k0 + nb,
-                                                                                               k0, below, nb);
-      SOV_LAUNCH_CHECK();
     }
-    timed_end(tp, s, below > 0 ? 3 : 1, 0.0);
+    timed_end(tp, s, below > 0 ? 2 : 1, 0.0);
     if (below <= 0) continue;
     // Look-ahead: the next panel's column strip first (on s, the critical
     // path), the rest of the trailing matrix on the side stream, where it
@@ -612,16 +612,25 @@ int DenseCholesky<T>::factor(T* a, int m, const double* diag0, hipStream_t s) {
     T* a_strip = a + (k0 + nb) + static_cast<long long>(k0 + nb) * lda;
     SOV_HIP_TRY((gemm_update<T, false, false>(below, strip, nb, L21, lda, L21, lda, a_strip, lda, T(-1), s)));
     timed_end(ts, s, 1, 0.0);
+    // Side stream: the bulk update of the rest, then L21 back into the
+    // factor (only the triangular solves read it there). ev_bulk_ marks the
+    // bulk update: the next strip waits for it. The L21 buffer of this panel
+    // is written again kPanelBuffers = 3 panels later, by which time the main
+    // stream has waited for the next panel's bulk update, queued after this
+    // copy on the side stream.
+    SOV_HIP_TRY(hipEventRecord(ev_panel_, s));
+    SOV_HIP_TRY(hipStreamWaitEvent(side_, ev_panel_, 0));
     if (rest > 0) {
-      SOV_HIP_TRY(hipEventRecord(ev_panel_, s));
-      SOV_HIP_TRY(hipStreamWaitEvent(side_, ev_panel_, 0));
       T* a22 = a + (k0 + nb + strip) + static_cast<long long>(k0 + nb + strip) * lda;
       const T* l21r = L21 + strip;
       TimedPair* tb = timed_begin(side_, sizeof(T) == 8 ? kTimeSyrk : kTimeSyrk32);
       SOV_HIP_TRY((gemm_update<T, false, true>(rest, rest, nb, l21r, lda, l21r, lda, a22, lda, T(-1), side_)));
       timed_end(tb, side_, 1, static_cast<double>(rest) * (rest + 1) * nb);
-      SOV_HIP_TRY(hipEventRecord(ev_bulk_, side_));
     }
+    SOV_HIP_TRY(hipEventRecord(ev_bulk_, side_));
+    copy_panel_kernel<T><<<blocks_for(static_cast<long long>(below) * nb, 256), 256, 0, side_>>>(L21, a, lda, k0 + nb,
+                                                                                                  k0, below, nb);
+    SOV_LAUNCH_CHECK();
   }
   SOV_HIP_TRY(hipEventRecord(ev_bulk_, side_));
   SOV_HIP_TRY(hipStreamWaitEvent(s, ev_bulk_, 0));
@@ -715,7 +724,7 @@ template class DenseCholesky<float>;
 
 extern "C" {
 
-void sov_set_kernel_timing(int32_t on) { sov::set_kernel_timing(on != 0); }
+void sov_set_kernel_timing(int32_t on) { sov::set_kernel_timing(on == 2 ? 2 : (on != 0 ? 1 : 0)); }
 
 int sov_kernel_times(double* out) {
   if (!out) return SOV_ERR_ARG;

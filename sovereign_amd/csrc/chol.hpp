@@ -62,7 +62,7 @@ struct KernelTimes {
   double strip_ms = 0.0, inverse_ms = 0.0;
   double syrk32_ms = 0.0, syrk32_launches = 0.0, syrk32_flops = 0.0;  // FP32 factorizations' SYRK
 };
-void set_kernel_timing(bool on);
+void set_kernel_timing(int level);
 bool kernel_timing();
 KernelTimes& kernel_times();
 void collect_kernel_times(hipStream_t s);
