@@ -55,8 +55,28 @@ import numpy as np  # noqa: E402
 
 METRIC = "IPM solve s & strong-branch LP solves/s at 1e-8 gap vs plain FP64 GPU path"
 MFMA_PEAK = {"f64": 78.6, "f32": 157.3}  # TFLOP/s, MI355X dense matrix peaks
-KERNEL_NAME = {"f64": "gemm_update_kernel<double, false, true, false>",
-               "f32": "gemm_update_kernel<float, false, true, false>"}
+SHAPES = {0: (64, 64), 1: (128, 64), 2: (128, 128)}
+
+
+def kernel_name(dtype, shape):
+    bm, bn = SHAPES.get(shape, (64, 64))
+    t = "double" if dtype == "f64" else "float"
+    return f"gemm_update_kernel<{t}, false, true, false, {bm}, {bn}>"
+
+
+def tuned_shapes(lib):
+    """Tile shapes the factorization's autotuner picked, and its timings."""
+    sh = np.full(4, -1, dtype=np.int32)
+    tm = np.zeros(10)
+    lib.sov_gemm_shapes(sh.ctypes.data_as(C.POINTER(C.c_int32)), tm.ctypes.data_as(C.POINTER(C.c_double)))
+    names = ["64x64", "128x64", "128x128"]
+    out = {}
+    for p, (dt, off) in enumerate((("f64", 0), ("f32", 5))):
+        out[dt] = {"syrk": names[sh[2 * p]] if sh[2 * p] >= 0 else None,
+                   "panel": names[sh[2 * p + 1]] if sh[2 * p + 1] >= 0 else None,
+                   "syrk_ms": {"64x64": tm[off], "128x128": tm[off + 1]},
+                   "panel_ms": {"64x64": tm[off + 2], "128x64": tm[off + 3], "128x128": tm[off + 4]}}
+    return out, sh
 NT = 12  # slots of sov_kernel_times
 
 
@@ -113,16 +133,17 @@ def load_traffic():
         return json.load(fh)
 
 
-def roofline(kt, dtype, traffic):
+def roofline(kt, dtype, traffic, shape=0):
     ms, launches, flops = (kt[0], kt[1], kt[2]) if dtype == "f64" else (kt[9], kt[10], kt[11])
     if ms <= 0 or launches <= 0:
         return None
     achieved = flops / (ms * 1e-3) / 1e12
-    tr = traffic.get(KERNEL_NAME[dtype])
+    name = kernel_name(dtype, shape)
+    tr = traffic.get(name)
     return {"bound": "mfma", "achieved": achieved, "peak": MFMA_PEAK[dtype], "unit": "TFLOP/s",
             "frac": achieved / MFMA_PEAK[dtype],
             "traffic": tr["bytes_per_launch"] if tr else None,
-            "kernel": KERNEL_NAME[dtype] + " (bulk SYRK trailing update)",
+            "kernel": name + " (bulk SYRK trailing update)",
             "launches": int(launches), "avg_launch_us": ms * 1e3 / launches,
             "flops_per_launch": flops / launches,
             "traffic_source": tr.get("source") if tr else None}
@@ -327,6 +348,7 @@ def main():
     main_path = "mixed" if "mixed" in runs else paths[0]
     res, wall, kt, _ = runs[main_path]
     traffic = load_traffic()
+    shapes_info, shape_ids = tuned_shapes(lib)
     # dominant MFMA kernel of the measured path: the SYRK precision with the most time
     dom = "f32" if kt[9] >= kt[0] else "f64"
     line = {
@@ -348,7 +370,8 @@ def main():
                            for p, rr in runs.items()},
         "resident_setup_s": setup_s,
         "kernel_ms_one_instrumented_solve": {p: kernel_ms(rr[3]) for p, rr in runs.items()},
-        "roofline": roofline(kt, dom, traffic),
+        "roofline": roofline(kt, dom, traffic, shape_ids[0 if dom == "f64" else 2]),
+        "autotuned_tiles": shapes_info,
     }
     if "mixed" in runs and "fp64" in runs:
         s_mx = runs["mixed"][1] / args.steps
@@ -360,7 +383,7 @@ def main():
             "objective_rel_diff_mixed_vs_fp64": abs(o_mx - o_64) / max(1.0, abs(o_64)),
             "pcg_iterations_mixed": runs["mixed"][0][-1]["refine_iterations"],
             "max_solve_residual_mixed": runs["mixed"][0][-1]["max_solve_residual"],
-            "roofline_fp64_path": roofline(runs["fp64"][2], "f64", traffic),
+            "roofline_fp64_path": roofline(runs["fp64"][2], "f64", traffic, shape_ids[0]),
         })
     if not args.no_branching:
         line["strong_branching"] = branching_leg(lib)

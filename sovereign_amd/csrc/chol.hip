@@ -20,6 +20,7 @@
 
 #include <algorithm>
 #include <atomic>
+#include <cstdlib>
 #include <vector>
 
 namespace sov {
@@ -567,6 +568,84 @@ int DenseCholesky<T>::reserve(int m) {
   return SOV_OK;
 }
 
+
+// ---------------------------------------------------------------------------
+// Tile-shape autotuner for the two MFMA products of the factorization: the
+// bulk trailing SYRK (lower, square tiles: 64x64 or 128x128) and the panel
+// products (TRSM as a GEMM and the strip update, M x 128 x 128: 64x64,
+// 128x64 or 128x128). On the first factorization in a precision each shape
+// runs a product of the class at a representative size (SYRK 4096 x 4096 x
+// 128, panel 8192 x 128 x 128), one warm-up and three timed launches (HIP
+// events), and the fastest is kept for the process. SOVEREIGN_GEMM_SHAPE =
+// 0 / 1 / 2 forces one shape (128x64 falls back to 64x64 for the SYRK).
+// ---------------------------------------------------------------------------
+struct TunedShapes {
+  std::atomic<int> syrk{-1}, panel{-1};
+  double syrk_ms[kGemmShapes] = {0, 0, 0}, panel_ms[kGemmShapes] = {0, 0, 0};
+};
+template <typename T>
+TunedShapes& tuned() {
+  static TunedShapes t;
+  return t;
+}
+
+int forced_shape() {
+  const char* e = std::getenv("SOVEREIGN_GEMM_SHAPE");
+  if (!e || !*e) return -1;
+  const int v = std::atoi(e);
+  return v >= 0 && v < kGemmShapes ? v : -1;
+}
+
+template <typename T>
+int autotune_shapes(hipStream_t s) {
+  TunedShapes& t = tuned<T>();
+  if (t.syrk.load() >= 0) return SOV_OK;
+  const int ms = 4096, mp = 8192, k = kPanel;
+  T *a = nullptr, *c = nullptr;
+  const std::size_t na = static_cast<std::size_t>(mp) * k, nc = static_cast<std::size_t>(ms) * ms;
+  SOV_HIP_TRY(hipMalloc(&a, sizeof(T) * na));
+  if (hipMalloc(&c, sizeof(T) * nc) != hipSuccess) {
+    (void)hipFree(a);
+    set_error("autotune: out of device memory");
+    return SOV_ERR_NOMEM;
+  }
+  (void)hipMemsetAsync(a, 0, sizeof(T) * na, s);
+  (void)hipMemsetAsync(c, 0, sizeof(T) * nc, s);
+  hipEvent_t e0, e1;
+  (void)hipEventCreate(&e0);
+  (void)hipEventCreate(&e1);
+  auto time_it = [&](auto&& launch) {
+    launch();
+    (void)hipEventRecord(e0, s);
+    for (int r = 0; r < 3; ++r) launch();
+    (void)hipEventRecord(e1, s);
+    (void)hipEventSynchronize(e1);
+    float ms_ = 0.f;
+    (void)hipEventElapsedTime(&ms_, e0, e1);
+    return static_cast<double>(ms_) / 3.0;
+  };
+  int best_s = kGemm64x64, best_p = kGemm64x64;
+  for (int shape : {kGemm64x64, kGemm128x128}) {
+    t.syrk_ms[shape] = time_it([&] {
+      (void)gemm_update_shape<T, false, true>(shape, ms, ms, k, a, ms, a, ms, c, ms, T(-1), s);
+    });
+    if (t.syrk_ms[shape] < t.syrk_ms[best_s]) best_s = shape;
+  }
+  for (int shape : {kGemm64x64, kGemm128x64, kGemm128x128}) {
+    t.panel_ms[shape] = time_it([&] {
+      (void)gemm_update_shape<T, false, false>(shape, mp, k, k, a, mp, a, mp, c, mp, T(-1), s);
+    });
+    if (t.panel_ms[shape] < t.panel_ms[best_p]) best_p = shape;
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  (void)hipFree(a);
+  (void)hipFree(c);
+  t.panel.store(best_p);
+  t.syrk.store(best_s);
+  return SOV_OK;
+}
+
 template <typename T>
 int DenseCholesky<T>::factor(T* a, int m, const double* diag0, hipStream_t s) {
   int rc = reserve(m);
@@ -576,6 +655,10 @@ int DenseCholesky<T>::factor(T* a, int m, const double* diag0, hipStream_t s) {
     SOV_HIP_TRY(hipEventCreateWithFlags(&ev_panel_, hipEventDisableTiming));
     SOV_HIP_TRY(hipEventCreateWithFlags(&ev_bulk_, hipEventDisableTiming));
   }
+  if ((rc = autotune_shapes<T>(s)) != SOV_OK) return rc;
+  const int forced = forced_shape();
+  const int syrk_shape = forced >= 0 ? forced : tuned<T>().syrk.load();
+  const int panel_shape = forced >= 0 ? forced : tuned<T>().panel.load();
   reset_flags_kernel<<<1, 1, 0, s>>>(flags_);
   SOV_LAUNCH_CHECK();
   // The side stream starts after everything already queued on s.
@@ -597,7 +680,8 @@ int DenseCholesky<T>::factor(T* a, int m, const double* diag0, hipStream_t s) {
       // L21 = A21 inv(L11)^T on MFMA, out of place (the updates read it
       // there; the copy back into the factor runs on the side stream).
       const T* a21 = a + (k0 + nb) + static_cast<long long>(k0) * lda;
-      SOV_HIP_TRY((gemm_update<T, false, false, true>(below, nb, nb, a21, lda, pinv, kSuper, L21, lda, T(1), s)));
+      SOV_HIP_TRY((gemm_update_shape<T, false, false, true>(panel_shape, below, nb, nb, a21, lda, pinv, kSuper, L21,
+                                                            lda, T(1), s)));
     }
     timed_end(tp, s, below > 0 ? 2 : 1, 0.0);
     if (below <= 0) continue;
@@ -610,7 +694,8 @@ int DenseCholesky<T>::factor(T* a, int m, const double* diag0, hipStream_t s) {
     SOV_HIP_TRY(hipStreamWaitEvent(s, ev_bulk_, 0));  // the strip was part of the previous bulk update
     TimedPair* ts = timed_begin(s, kTimeStrip);
     T* a_strip = a + (k0 + nb) + static_cast<long long>(k0 + nb) * lda;
-    SOV_HIP_TRY((gemm_update<T, false, false>(below, strip, nb, L21, lda, L21, lda, a_strip, lda, T(-1), s)));
+    SOV_HIP_TRY((gemm_update_shape<T, false, false>(panel_shape, below, strip, nb, L21, lda, L21, lda, a_strip, lda,
+                                                    T(-1), s)));
     timed_end(ts, s, 1, 0.0);
     // Side stream: the bulk update of the rest, then L21 back into the
     // factor (only the triangular solves read it there). ev_bulk_ marks the
@@ -624,7 +709,8 @@ int DenseCholesky<T>::factor(T* a, int m, const double* diag0, hipStream_t s) {
       T* a22 = a + (k0 + nb + strip) + static_cast<long long>(k0 + nb + strip) * lda;
       const T* l21r = L21 + strip;
       TimedPair* tb = timed_begin(side_, sizeof(T) == 8 ? kTimeSyrk : kTimeSyrk32);
-      SOV_HIP_TRY((gemm_update<T, false, true>(rest, rest, nb, l21r, lda, l21r, lda, a22, lda, T(-1), side_)));
+      SOV_HIP_TRY((gemm_update_shape<T, false, true>(syrk_shape, rest, rest, nb, l21r, lda, l21r, lda, a22, lda, T(-1),
+                                                     side_)));
       timed_end(tb, side_, 1, static_cast<double>(rest) * (rest + 1) * nb);
     }
     SOV_HIP_TRY(hipEventRecord(ev_bulk_, side_));
@@ -723,6 +809,23 @@ template class DenseCholesky<float>;
 }  // namespace sov
 
 extern "C" {
+
+int sov_gemm_shapes(int32_t* shapes4, double* times_ms10) {
+  if (!shapes4) return SOV_ERR_ARG;
+  shapes4[0] = sov::tuned<double>().syrk.load();
+  shapes4[1] = sov::tuned<double>().panel.load();
+  shapes4[2] = sov::tuned<float>().syrk.load();
+  shapes4[3] = sov::tuned<float>().panel.load();
+  if (times_ms10) {  // f64 SYRK 64x64, 128x128; f64 panel 64x64, 128x64, 128x128; the same for f32
+    const sov::TunedShapes* t[2] = {&sov::tuned<double>(), &sov::tuned<float>()};
+    for (int p = 0; p < 2; ++p) {
+      times_ms10[5 * p + 0] = t[p]->syrk_ms[sov::kGemm64x64];
+      times_ms10[5 * p + 1] = t[p]->syrk_ms[sov::kGemm128x128];
+      for (int k = 0; k < 3; ++k) times_ms10[5 * p + 2 + k] = t[p]->panel_ms[k];
+    }
+  }
+  return SOV_OK;
+}
 
 void sov_set_kernel_timing(int32_t on) { sov::set_kernel_timing(on == 2 ? 2 : (on != 0 ? 1 : 0)); }
 

@@ -9,12 +9,12 @@
 // rank-K update of a lower triangle) and launches only tiles on or below the
 // diagonal.
 //
-// Tiling for CDNA4: a 256-thread workgroup owns a 64x64 tile of C as four
-// wavefronts of 32x32, each a 2x2 grid of 16x16 MFMA blocks
-// (v_mfma_f64_16x16x4f64 / v_mfma_f32_16x16x4f32). K is staged through LDS in
-// chunks of 32 with a register prefetch of the next chunk. LDS rows are padded
-// to 80 elements so the four k-rows one MFMA operand read touches fall in
-// different banks. Workgroups are renumbered so each XCD (round-robin
+// Tiling for CDNA4: a 256-thread workgroup owns a BM x BN tile of C (64x64,
+// 128x64 or 128x128) as a 2x2 grid of wavefronts, each a grid of 16x16 MFMA
+// blocks (v_mfma_f64_16x16x4f64 / v_mfma_f32_16x16x4f32). K is staged through
+// LDS in chunks of 32 with a register prefetch of the next chunk. LDS rows are
+// padded by 16 elements so the four k-rows one MFMA operand read touches fall
+// in different banks. Workgroups are renumbered so each XCD (round-robin
 // dispatch over 8) works on a contiguous run of tiles, which share operand
 // rows in that XCD's L2.
 #pragma once
@@ -52,7 +52,6 @@ struct Mfma<float> {
 
 constexpr int kTile = 64;
 constexpr int kChunk = 32;
-constexpr int kLdsStride = 80;
 
 // Block index with XCD-contiguous renumbering (bijective on the grid).
 __device__ __forceinline__ int xcd_swizzle(int b, int grid) {
@@ -80,49 +79,56 @@ __device__ __forceinline__ void tile_coords(int t, bool lower, int tiles_m, int*
   }
 }
 
-template <typename T, bool NN, bool LOWER, bool STORE = false>
+template <typename T, bool NN, bool LOWER, bool STORE = false, int BM = kTile, int BN = kTile>
 __global__ __launch_bounds__(256) void gemm_update_kernel(int M, int N, int K, const T* __restrict__ A,
                                                           long long lda, const T* __restrict__ B,
                                                           long long ldb, T* __restrict__ C,
                                                           long long ldc, T alpha, GemmBatch bt) {
+  static_assert(BM % 32 == 0 && BN % 32 == 0 && BM <= 128 && BN <= 128, "tile shape");
+  static_assert(!LOWER || BM == BN, "triangular tiling needs square tiles");
   typedef typename Mfma<T>::acc_t acc_t;
+  constexpr int IM = BM / 32, JN = BN / 32;          // 16 x 16 blocks per wavefront (2 x 2 wavefronts)
+  constexpr int LA = BM + 16, LB = BN + 16;          // padded LDS rows
+  constexpr int QA = BM * kChunk / 256, QB = BN * kChunk / 256;  // staged elements per thread
   A += blockIdx.y * bt.a_y + blockIdx.z * bt.a_z;
   B += blockIdx.y * bt.b_y + blockIdx.z * bt.b_z;
   C += blockIdx.y * bt.c_y + blockIdx.z * bt.c_z;
-  __shared__ T As[kChunk * kLdsStride];
-  __shared__ T Bs[kChunk * kLdsStride];
+  __shared__ T As[kChunk * LA];
+  __shared__ T Bs[kChunk * LB];
 
-  const int tiles_m = (M + kTile - 1) / kTile;
+  const int tiles_m = (M + BM - 1) / BM;
   int ti, tj;
   tile_coords(xcd_swizzle(blockIdx.x, gridDim.x), LOWER, tiles_m, &ti, &tj);
-  const int r0 = ti * kTile, c0 = tj * kTile;
+  const int r0 = ti * BM, c0 = tj * BN;
 
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6;
-  const int wr = (wave & 1) * 32, wc = (wave >> 1) * 32;
+  const int wr = (wave & 1) * (BM / 2), wc = (wave >> 1) * (BN / 2);
   const int lrow = lane & 15, lk = lane >> 4;
 
-  acc_t acc[2][2];
-  for (int i = 0; i < 2; ++i)
-    for (int j = 0; j < 2; ++j) acc[i][j] = acc_t{0, 0, 0, 0};
+  acc_t acc[IM][JN];
+#pragma unroll
+  for (int i = 0; i < IM; ++i)
+#pragma unroll
+    for (int j = 0; j < JN; ++j) acc[i][j] = acc_t{0, 0, 0, 0};
 
-  // Global -> register staging of one chunk: 8 elements of A and of B per thread.
-  T ra[8], rb[8];
+  // Global -> register staging of one chunk of kChunk k-columns.
+  T ra[QA], rb[QB];
   auto load_chunk = [&](int k0) {
 #pragma unroll
-    for (int q = 0; q < 8; ++q) {
-      const int r = tid & 63, k = (tid >> 6) + 4 * q;
+    for (int q = 0; q < QA; ++q) {
+      const int r = tid % BM, k = tid / BM + (256 / BM) * q;
       const int gr = r0 + r, gk = k0 + k;
       ra[q] = (gr < M && gk < K) ? A[gr + static_cast<long long>(gk) * lda] : T(0);
     }
 #pragma unroll
-    for (int q = 0; q < 8; ++q) {
+    for (int q = 0; q < QB; ++q) {
       if (NN) {
-        const int k = tid & 31, c = (tid >> 5) + 8 * q;
+        const int k = tid % kChunk, c = tid / kChunk + (256 / kChunk) * q;
         const int gc = c0 + c, gk = k0 + k;
         rb[q] = (gc < N && gk < K) ? B[gk + static_cast<long long>(gc) * ldb] : T(0);
       } else {
-        const int c = tid & 63, k = (tid >> 6) + 4 * q;
+        const int c = tid % BN, k = tid / BN + (256 / BN) * q;
         const int gc = c0 + c, gk = k0 + k;
         rb[q] = (gc < N && gk < K) ? B[gc + static_cast<long long>(gk) * ldb] : T(0);
       }
@@ -130,11 +136,11 @@ __global__ __launch_bounds__(256) void gemm_update_kernel(int M, int N, int K, c
   };
   auto store_chunk = [&]() {
 #pragma unroll
-    for (int q = 0; q < 8; ++q) As[((tid >> 6) + 4 * q) * kLdsStride + (tid & 63)] = ra[q];
+    for (int q = 0; q < QA; ++q) As[(tid / BM + (256 / BM) * q) * LA + tid % BM] = ra[q];
 #pragma unroll
-    for (int q = 0; q < 8; ++q) {
-      if (NN) Bs[(tid & 31) * kLdsStride + (tid >> 5) + 8 * q] = rb[q];
-      else Bs[((tid >> 6) + 4 * q) * kLdsStride + (tid & 63)] = rb[q];
+    for (int q = 0; q < QB; ++q) {
+      if (NN) Bs[(tid % kChunk) * LB + tid / kChunk + (256 / kChunk) * q] = rb[q];
+      else Bs[(tid / BN + (256 / BN) * q) * LB + tid % BN] = rb[q];
     }
   };
 
@@ -146,15 +152,18 @@ __global__ __launch_bounds__(256) void gemm_update_kernel(int M, int N, int K, c
     if (k0 + kChunk < K) load_chunk(k0 + kChunk);
 #pragma unroll
     for (int kk = 0; kk < kChunk; kk += 4) {
-      const int krow = (kk + lk) * kLdsStride;
-      const T a0 = As[krow + wr + lrow], a1 = As[krow + wr + 16 + lrow];
-      const T b0 = Bs[krow + wc + lrow], b1 = Bs[krow + wc + 16 + lrow];
+      const int ka = (kk + lk) * LA, kb = (kk + lk) * LB;
+      T av[IM], bv[JN];
+#pragma unroll
+      for (int i = 0; i < IM; ++i) av[i] = As[ka + wr + 16 * i + lrow];
+#pragma unroll
+      for (int j = 0; j < JN; ++j) bv[j] = Bs[kb + wc + 16 * j + lrow];
       // The B tile goes in as the instruction's first operand, so each
       // 16 x 16 result comes out transposed: a lane holds one row of C.
-      acc[0][0] = Mfma<T>::step(b0, a0, acc[0][0]);
-      acc[0][1] = Mfma<T>::step(b1, a0, acc[0][1]);
-      acc[1][0] = Mfma<T>::step(b0, a1, acc[1][0]);
-      acc[1][1] = Mfma<T>::step(b1, a1, acc[1][1]);
+#pragma unroll
+      for (int i = 0; i < IM; ++i)
+#pragma unroll
+        for (int j = 0; j < JN; ++j) acc[i][j] = Mfma<T>::step(bv[j], av[i], acc[i][j]);
     }
   }
 
@@ -165,11 +174,11 @@ __global__ __launch_bounds__(256) void gemm_update_kernel(int M, int N, int K, c
   // four C columns: one store instruction writes 16 consecutive rows of each
   // of four columns.
 #pragma unroll
-  for (int i = 0; i < 2; ++i) {
+  for (int i = 0; i < IM; ++i) {
     const int gr = r0 + wr + 16 * i + lrow;
     if (gr >= M) continue;
 #pragma unroll
-    for (int j = 0; j < 2; ++j) {
+    for (int j = 0; j < JN; ++j) {
 #pragma unroll
       for (int g = 0; g < 4; ++g) {
         const int gc = c0 + wc + 16 * j + Mfma<T>::row(lk, g);
@@ -181,18 +190,36 @@ __global__ __launch_bounds__(256) void gemm_update_kernel(int M, int N, int K, c
   }
 }
 
-// Host launcher: one product, or ny x nz of them offset by `bt`. Returns a
-// hipError_t from the launch.
-template <typename T, bool NN, bool LOWER, bool STORE = false>
+// Host launcher: one product, or ny x nz of them offset by `bt`, with BM x BN
+// tiles. Returns a hipError_t from the launch.
+template <typename T, bool NN, bool LOWER, bool STORE = false, int BM = kTile, int BN = kTile>
 inline hipError_t gemm_update(int M, int N, int K, const T* A, long long lda, const T* B, long long ldb, T* C,
                               long long ldc, T alpha, hipStream_t s, int ny = 1, int nz = 1,
                               GemmBatch bt = GemmBatch{}) {
   if (M <= 0 || N <= 0 || K <= 0 || ny <= 0 || nz <= 0) return hipSuccess;
-  const long long tm = (M + kTile - 1) / kTile, tn = (N + kTile - 1) / kTile;
+  const long long tm = (M + BM - 1) / BM, tn = (N + BN - 1) / BN;
   const long long tiles = LOWER ? tm * (tm + 1) / 2 : tm * tn;
   const dim3 grid(static_cast<unsigned>(tiles), static_cast<unsigned>(ny), static_cast<unsigned>(nz));
-  gemm_update_kernel<T, NN, LOWER, STORE><<<grid, 256, 0, s>>>(M, N, K, A, lda, B, ldb, C, ldc, alpha, bt);
+  gemm_update_kernel<T, NN, LOWER, STORE, BM, BN><<<grid, 256, 0, s>>>(M, N, K, A, lda, B, ldb, C, ldc, alpha, bt);
   return hipGetLastError();
+}
+
+// Tile shapes the autotuner chooses between for a given product class.
+enum GemmShape { kGemm64x64 = 0, kGemm128x64 = 1, kGemm128x128 = 2, kGemmShapes = 3 };
+
+// gemm_update with a run-time tile shape (square shapes only when LOWER).
+template <typename T, bool NN, bool LOWER, bool STORE = false>
+inline hipError_t gemm_update_shape(int shape, int M, int N, int K, const T* A, long long lda, const T* B,
+                                    long long ldb, T* C, long long ldc, T alpha, hipStream_t s) {
+  switch (shape) {
+    case kGemm128x128:
+      return gemm_update<T, NN, LOWER, STORE, 128, 128>(M, N, K, A, lda, B, ldb, C, ldc, alpha, s);
+    case kGemm128x64:
+      if constexpr (!LOWER) return gemm_update<T, NN, LOWER, STORE, 128, 64>(M, N, K, A, lda, B, ldb, C, ldc, alpha, s);
+      else return gemm_update<T, NN, LOWER, STORE>(M, N, K, A, lda, B, ldb, C, ldc, alpha, s);
+    default:
+      return gemm_update<T, NN, LOWER, STORE>(M, N, K, A, lda, B, ldb, C, ldc, alpha, s);
+  }
 }
 
 }  // namespace sov

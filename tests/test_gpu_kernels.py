@@ -114,6 +114,43 @@ def test_dense_lu_reports_singular(gpu_api):
     assert gpu_api.dense_lu(np.zeros((3, 3)))[0] == 1
 
 
+@pytest.mark.parametrize("shape", [0, 1, 2])
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_normal_equations_each_tile_shape(gpu_api, monkeypatch, shape, precision):
+    """Every MFMA tile shape the autotuner can pick (forced through
+    SOVEREIGN_GEMM_SHAPE) gives the oracle's solution; m = 1000 leaves
+    partial tiles at every edge."""
+    lp = P.random_sparse_lp(1000, 4000, 6, seed=5)
+    rng = np.random.default_rng(11)
+    d = np.exp(rng.uniform(-3, 3, size=lp.n))
+    rhs = O.spmv(lp.m, lp.n, lp.col_ptr, lp.row_idx, lp.values, rng.standard_normal(lp.n))
+    ne_c = O.NormalEquations(lp.m, lp.n, lp.col_ptr, lp.row_idx, lp.values, "dense")
+    assert ne_c.factor(d)
+    at_c = O.spmv(lp.m, lp.n, lp.col_ptr, lp.row_idx, lp.values, ne_c.solve(rhs), True)
+    monkeypatch.setenv("SOVEREIGN_GEMM_SHAPE", str(shape))
+    say(f"normal equations tile shape {shape} {precision}")
+    A = gpu_api.SparseMatrix(lp.m, lp.n, lp.col_ptr, lp.row_idx, lp.values)
+    ne = gpu_api.NormalEquations(A)
+    assert ne.factor(d, precision)
+    at_g = O.spmv(lp.m, lp.n, lp.col_ptr, lp.row_idx, lp.values, ne.solve(rhs), True)
+    ne.close()
+    np.testing.assert_allclose(at_g, at_c, rtol=0, atol=1e-7 * max(1.0, np.max(np.abs(at_c))))
+
+
+def test_autotuner_reports_shapes(gpu_api):
+    """After factorizations in both precisions the tuner has chosen a shape
+    per product class and timed every candidate."""
+    import ctypes as C
+
+    from sovereign_amd import _capi
+
+    shapes = np.full(4, -2, dtype=np.int32)
+    times = np.zeros(10)
+    _capi.load().sov_gemm_shapes(shapes.ctypes.data_as(C.POINTER(C.c_int32)), times.ctypes.data_as(C.POINTER(C.c_double)))
+    assert set(shapes[[0, 2]]) <= {0, 2} and set(shapes[[1, 3]]) <= {0, 1, 2}, shapes
+    assert np.all(times > 0), times
+
+
 @pytest.mark.parametrize("case", ["random_300x1200", "random_1000x4000", "transport_50x50", "plan_600x900"])
 @pytest.mark.parametrize("precision", ["fp64", "fp32"])
 def test_normal_equations_solve(gpu_api, case, precision):
