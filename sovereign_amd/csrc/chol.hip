@@ -575,8 +575,8 @@ int DenseCholesky<T>::reserve(int m) {
 // products (TRSM as a GEMM and the strip update, M x 128 x 128: 64x64,
 // 128x64 or 128x128). On the first factorization in a precision each shape
 // runs a product of the class at a representative size (SYRK 4096 x 4096 x
-// 128, panel 8192 x 128 x 128), one warm-up and three timed launches (HIP
-// events), and the fastest is kept for the process. SOVEREIGN_GEMM_SHAPE =
+// 128, panel 8192 x 128 x 128), one warm-up and five timed launches (HIP
+// events, the fastest counts), and the fastest shape is kept for the process. SOVEREIGN_GEMM_SHAPE =
 // 0 / 1 / 2 forces one shape (128x64 falls back to 64x64 for the SYRK).
 // ---------------------------------------------------------------------------
 struct TunedShapes {
@@ -614,15 +614,19 @@ int autotune_shapes(hipStream_t s) {
   hipEvent_t e0, e1;
   (void)hipEventCreate(&e0);
   (void)hipEventCreate(&e1);
-  auto time_it = [&](auto&& launch) {
+  auto time_it = [&](auto&& launch) {  // fastest of five launches after a warm-up
     launch();
-    (void)hipEventRecord(e0, s);
-    for (int r = 0; r < 3; ++r) launch();
-    (void)hipEventRecord(e1, s);
-    (void)hipEventSynchronize(e1);
-    float ms_ = 0.f;
-    (void)hipEventElapsedTime(&ms_, e0, e1);
-    return static_cast<double>(ms_) / 3.0;
+    double best = 1e30;
+    for (int r = 0; r < 5; ++r) {
+      (void)hipEventRecord(e0, s);
+      launch();
+      (void)hipEventRecord(e1, s);
+      (void)hipEventSynchronize(e1);
+      float ms_ = 0.f;
+      (void)hipEventElapsedTime(&ms_, e0, e1);
+      best = std::min(best, static_cast<double>(ms_));
+    }
+    return best;
   };
   int best_s = kGemm64x64, best_p = kGemm64x64;
   for (int shape : {kGemm64x64, kGemm128x128}) {
