@@ -233,13 +233,17 @@ int sov_lp_solve(const sov_lp_model* model, const sov_ipm_options* opt, sov_ipm_
  * u == 0: x fixed at 0). Matrices are uploaded once and shared by LPs;
  * per-LP b lives at b_all + b_off[k] (m entries), c / u / x at
  * + x_off[k] (n entries). status: 0 optimal, 5 iteration limit,
- * 6 numerical breakdown. The objective is c'x of the given data. */
+ * 6 numerical breakdown. The objective is c'x of the given data. Each LP's
+ * normal matrix is factored in FP32 (LDS) as the preconditioner of FP64 CG;
+ * an LP whose solves stall continues with FP64 factorizations in global
+ * memory. */
 typedef struct sov_batch sov_batch;
 typedef struct {
   int32_t status;
   int32_t iterations;
   int32_t pcg_iterations;
-  int32_t pad;
+  int32_t fp64_from; /* iteration from which the LP's factorizations were FP64 (its FP32-preconditioned
+                      * solves had stalled), or -1 */
   double objective;
   double duality_gap;
   double primal_residual;

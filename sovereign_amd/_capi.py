@@ -80,7 +80,7 @@ class BatchResult(C.Structure):
         ("status", C.c_int32),
         ("iterations", C.c_int32),
         ("pcg_iterations", C.c_int32),
-        ("pad", C.c_int32),
+        ("fp64_from", C.c_int32),
         ("objective", C.c_double),
         ("duality_gap", C.c_double),
         ("primal_residual", C.c_double),

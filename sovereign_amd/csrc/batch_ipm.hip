@@ -29,6 +29,7 @@
 #include <vector>
 
 typedef float flt4_t __attribute__((ext_vector_type(4)));
+typedef double dbl4_t __attribute__((ext_vector_type(4)));
 
 namespace {
 
@@ -87,6 +88,32 @@ struct Lp {
   // workspace vectors (global, this LP only)
   double *x, *w, *z, *v, *rc, *ru, *D, *dx, *dw, *dz, *dv, *dxa, *dwa, *dza, *dva, *t;
   double *y, *rb, *dy, *r, *p, *q, *zz, *rhs, *sc;  // sc: Jacobi scale of A D A^T
+  double* M64;  // packed lower FP64 normal matrix (global), for LPs whose FP32 solves stall
+};
+
+// Storage of the packed normal matrix and its factor: FP32 in LDS, or FP64
+// in global memory for an LP that has switched to FP64 factorizations.
+template <typename T>
+struct Prec;
+template <>
+struct Prec<float> {
+  static constexpr float drop_pivot = 1e18f;
+  __device__ static __forceinline__ flt4_t mfma(float a, float b, flt4_t c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+  typedef flt4_t acc_t;
+  __device__ static __forceinline__ int row(int lk, int g) { return 4 * lk + g; }  // output row of register g
+  __device__ static __forceinline__ float a(const Lp& L, long long idx) { return L.Af[idx]; }
+};
+template <>
+struct Prec<double> {
+  static constexpr double drop_pivot = 1e64;
+  __device__ static __forceinline__ dbl4_t mfma(double a, double b, dbl4_t c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+  typedef dbl4_t acc_t;
+  __device__ static __forceinline__ int row(int lk, int g) { return lk + 4 * g; }
+  __device__ static __forceinline__ double a(const Lp& L, long long idx) { return L.A[idx]; }
 };
 
 // out_j = (A^T vec)_j for j < n; vec (m) is staged in LDS first.
@@ -139,7 +166,8 @@ __device__ void gemv_n(const Lp& L, const double* vec, const double* base, doubl
 // assembled on MFMA from the operands A(i, k) sqrt(D_k) sc_i, so Mt has a
 // unit diagonal whatever the spread of D; the FP32 factorization of Mt is
 // then accurate relative to its own scale. L.t holds sqrt(D) meanwhile.
-__device__ void assemble(const Lp& L, float* M, double reg) {
+template <typename T>
+__device__ void assemble(const Lp& L, T* M, double reg) {
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   const int lrow = lane & 15, lk = lane >> 4;
   for (int j = tid; j < L.n; j += kThreads) L.t[j] = sqrt(fmax(L.D[j], 0.0));
@@ -160,8 +188,8 @@ __device__ void assemble(const Lp& L, float* M, double reg) {
     }
   }
   __syncthreads();
-  const int T = (L.m + 15) / 16;
-  const int tiles = T * (T + 1) / 2;
+  const int nt = (L.m + 15) / 16;
+  const int tiles = nt * (nt + 1) / 2;
   for (int t = wv; t < tiles; t += kWaves) {
     int ti = static_cast<int>((sqrt(8.0 * t + 1.0) - 1.0) * 0.5);
     while ((ti + 1) * (ti + 2) / 2 <= t) ++ti;
@@ -170,26 +198,26 @@ __device__ void assemble(const Lp& L, float* M, double reg) {
     const int r0 = ti * 16, c0 = tj * 16;
     const int ra = r0 + lrow, cb = c0 + lrow;
     const double sa = ra < L.m ? L.sc[ra] : 0.0, sbv = cb < L.m ? L.sc[cb] : 0.0;
-    flt4_t acc = {0.f, 0.f, 0.f, 0.f};
+    typename Prec<T>::acc_t acc = {0, 0, 0, 0};
     for (int k0 = 0; k0 < L.n; k0 += 16) {
-      float av[4], bv[4];
+      T av[4], bv[4];
 #pragma unroll
       for (int u = 0; u < 4; ++u) {
         const int k = k0 + 4 * u + lk;
         const bool kin = k < L.n;
         const double sd = kin ? L.t[k] : 0.0;
-        av[u] = (kin && ra < L.m) ? L.Af[ra + static_cast<long long>(k) * L.m] * static_cast<float>(sd * sa) : 0.f;
-        bv[u] = (kin && cb < L.m) ? L.Af[cb + static_cast<long long>(k) * L.m] * static_cast<float>(sd * sbv) : 0.f;
+        av[u] = (kin && ra < L.m) ? Prec<T>::a(L, ra + static_cast<long long>(k) * L.m) * static_cast<T>(sd * sa) : T(0);
+        bv[u] = (kin && cb < L.m) ? Prec<T>::a(L, cb + static_cast<long long>(k) * L.m) * static_cast<T>(sd * sbv) : T(0);
       }
 #pragma unroll
-      for (int u = 0; u < 4; ++u) acc = __builtin_amdgcn_mfma_f32_16x16x4f32(av[u], bv[u], acc, 0, 0, 0);
+      for (int u = 0; u < 4; ++u) acc = Prec<T>::mfma(av[u], bv[u], acc);
     }
-    // lane holds column c0 + lrow, rows r0 + 4*lk + g
+    // lane holds column c0 + lrow, rows r0 + Prec<T>::row(lk, g)
 #pragma unroll
     for (int g = 0; g < 4; ++g) {
-      const int i = r0 + 4 * lk + g, j = c0 + lrow;
+      const int i = r0 + Prec<T>::row(lk, g), j = c0 + lrow;
       if (i < L.m && j < L.m && i >= j)
-        M[packed_index(i, j, L.m)] = acc[g] + (i == j ? static_cast<float>(reg * L.sc[i] * L.sc[i]) : 0.f);
+        M[packed_index(i, j, L.m)] = acc[g] + (i == j ? static_cast<T>(reg * L.sc[i] * L.sc[i]) : T(0));
     }
   }
   __syncthreads();
@@ -198,7 +226,8 @@ __device__ void assemble(const Lp& L, float* M, double reg) {
 // In-place FP32 Cholesky of the packed lower triangle in LDS (unit diagonal
 // before elimination, so the reference's drop rule reads pivot <= 1e-13).
 // Returns the number of dropped pivots.
-__device__ int cholesky(const Lp& L, float* M) {
+template <typename T>
+__device__ int cholesky(const Lp& L, T* M) {
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   const int m = L.m;
   int dropped = 0;
@@ -206,13 +235,13 @@ __device__ int cholesky(const Lp& L, float* M) {
     const long long kk = packed_index(k, k, m);
     const double dk = static_cast<double>(M[kk]);
     const bool drop = !(dk > 1e-13) && isfinite(dk);
-    const float piv = drop ? 1e18f : (isfinite(dk) ? static_cast<float>(sqrt(dk)) : 1.f);
+    const T piv = drop ? Prec<T>::drop_pivot : (isfinite(dk) ? static_cast<T>(sqrt(dk)) : T(1));
     dropped += drop ? 1 : 0;
     for (int i = k + 1 + tid; i < m; i += kThreads) M[kk + (i - k)] /= piv;
     __syncthreads();
     if (tid == 0) M[kk] = piv;
     for (int j = k + 1 + wv; j < m; j += kWaves) {
-      const float ljk = M[kk + (j - k)];
+      const T ljk = M[kk + (j - k)];
       const long long cj = packed_index(j, j, m);
       for (int i = j + lane; i < m; i += 64) M[cj + (i - j)] -= M[kk + (i - k)] * ljk;
     }
@@ -226,15 +255,16 @@ __device__ int cholesky(const Lp& L, float* M) {
 // L(k, j), with the columns right of j already inverted. Four threads per
 // row share the sum; tmp (LDS, m doubles) holds the column until all reads
 // of L(., j) are done.
-__device__ void invert_lower(const Lp& L, float* M, double* tmp) {
+template <typename T>
+__device__ void invert_lower(const Lp& L, T* M, double* tmp) {
   const int tid = threadIdx.x, m = L.m;
   const int row = tid >> 2, part = tid & 3;
   for (int j = m - 1; j >= 0; --j) {
     const long long jj = packed_index(j, j, m);
-    const float xjj = 1.f / M[jj];  // read before anyone rewrites it below
+    const T xjj = T(1) / M[jj];  // read before anyone rewrites it below
     for (int i0 = j + 1; i0 < m; i0 += kThreads / 4) {
       const int i = i0 + row;
-      float acc = 0.f;
+      T acc = T(0);
       if (i < m) {
         // X(i, k) at packed_index(i, k): the stride to k + 1 is m - k - 1
         long long idx = packed_index(i, j + 1 + part, m);
@@ -248,7 +278,7 @@ __device__ void invert_lower(const Lp& L, float* M, double* tmp) {
       if (part == 0 && i < m) tmp[i] = acc;
     }
     __syncthreads();
-    for (int i = j + 1 + tid; i < m; i += kThreads) M[jj + (i - j)] = -xjj * static_cast<float>(tmp[i]);
+    for (int i = j + 1 + tid; i < m; i += kThreads) M[jj + (i - j)] = -xjj * static_cast<T>(tmp[i]);
     if (tid == 0) M[jj] = xjj;
     __syncthreads();
   }
@@ -256,7 +286,8 @@ __device__ void invert_lower(const Lp& L, float* M, double* tmp) {
 
 // vec <- M^-1 vec = S X^T X S vec (vec in LDS, FP64), with X = inverse of
 // the FP32 factor of the scaled matrix; tmp (LDS, m doubles) is scratch.
-__device__ void precondition(const Lp& L, const float* M, double* vec, double* tmp) {
+template <typename T>
+__device__ void precondition(const Lp& L, const T* M, double* vec, double* tmp) {
   const int tid = threadIdx.x, m = L.m;
   const int row = tid >> 2, part = tid & 3;
   __syncthreads();
@@ -300,7 +331,8 @@ __device__ void normal_matvec(const Lp& L, const double* p, double* q, double* s
 }
 
 // dy = M^-1 rhs by PCG. Returns iterations.
-__device__ int solve_normal(const Lp& L, const float* M, double* stage, double* red, double tol, int max_it) {
+template <typename T>
+__device__ int solve_normal(const Lp& L, const T* M, double* stage, double* red, double tol, int max_it) {
   const int tid = threadIdx.x, m = L.m;
   double* pv = stage;       // [0, m)  preconditioner vector (LDS)
   double* tv = stage + 512;  // [512, 512+m) preconditioner temp (LDS)
@@ -364,8 +396,9 @@ __device__ __forceinline__ bool bounded(const Lp& L, int j) { return L.u[j] < 1e
 
 // Newton direction for complementarity right-hand sides r1 (x z) and r2 (w v),
 // given the current residuals; writes (dx, dw, dz, dv) and dy.
-__device__ int newton(const Lp& L, const float* M, double* stage, double* red, const double* r1, const double* r2,
-                      double* dx, double* dw, double* dz, double* dv, const BatchOpts& o) {
+template <typename T>
+__device__ int newton(const Lp& L, const T* M, double* stage, double* red, const double* r1, const double* r2,
+                      double* dx, double* dw, double* dz, double* dv, double pcg_tol, const BatchOpts& o) {
   const int tid = threadIdx.x;
   for (int j = tid; j < L.n; j += kThreads) {
     double tj = 0.0;
@@ -381,7 +414,7 @@ __device__ int newton(const Lp& L, const float* M, double* stage, double* red, c
   // keep t: gemv_t / normal_matvec overwrite L.t, so stash it in dx first
   for (int j = tid; j < L.n; j += kThreads) dx[j] = L.t[j];
   __syncthreads();
-  const int its = solve_normal(L, M, stage, red, o.pcg_tol, o.pcg_max);
+  const int its = solve_normal(L, M, stage, red, pcg_tol, o.pcg_max);
   gemv_t(L, L.dy, L.t, stage);  // t = A^T dy
   for (int j = tid; j < L.n; j += kThreads) {
     if (!active(L, j)) {
@@ -432,6 +465,7 @@ __global__ __launch_bounds__(kThreads) void batch_ipm_kernel(const BatchMatrix* 
   double* red = stage + kStage;                              // 64 doubles
   float* M = reinterpret_cast<float*>(red + 64);             // packed lower triangle
   __shared__ int s_lp;
+  __shared__ Lp s_L;
   const int tid = threadIdx.x;
   for (;;) {
     __syncthreads();
@@ -441,26 +475,34 @@ __global__ __launch_bounds__(kThreads) void batch_ipm_kernel(const BatchMatrix* 
     if (lp >= count) break;
     const BatchDesc dsc = descs[lp];
     const BatchMatrix mat = mats[dsc.mat];
-    Lp L;
-    L.m = mat.m;
-    L.n = mat.n;
-    L.A = mat.a64;
-    L.Af = mat.a32;
-    L.b = b_all + dsc.b_off;
-    L.c = c_all + dsc.x_off;
-    L.u = u_all + dsc.x_off;
-    double* wsp = ws + dsc.ws_off;
-    double** nv[] = {&L.x, &L.w, &L.z, &L.v, &L.rc, &L.ru, &L.D, &L.dx, &L.dw, &L.dz, &L.dv,
-                     &L.dxa, &L.dwa, &L.dza, &L.dva, &L.t};
-    for (double** pp : nv) {
-      *pp = wsp;
-      wsp += L.n;
+    // The LP's sizes and pointers live in LDS: every thread reads the same
+    // values, and keeping them out of registers leaves those to the loops.
+    __syncthreads();
+    if (tid == 0) {
+      Lp& W = s_L;
+      W.m = mat.m;
+      W.n = mat.n;
+      W.A = mat.a64;
+      W.Af = mat.a32;
+      W.b = b_all + dsc.b_off;
+      W.c = c_all + dsc.x_off;
+      W.u = u_all + dsc.x_off;
+      double* wsp = ws + dsc.ws_off;
+      double** nv[] = {&W.x, &W.w, &W.z, &W.v, &W.rc, &W.ru, &W.D, &W.dx, &W.dw, &W.dz, &W.dv,
+                       &W.dxa, &W.dwa, &W.dza, &W.dva, &W.t};
+      for (double** pp : nv) {
+        *pp = wsp;
+        wsp += W.n;
+      }
+      double** mv[] = {&W.y, &W.rb, &W.dy, &W.r, &W.p, &W.q, &W.zz, &W.rhs, &W.sc};
+      for (double** pp : mv) {
+        *pp = wsp;
+        wsp += W.m;
+      }
+      W.M64 = wsp;  // m (m + 1) / 2
     }
-    double** mv[] = {&L.y, &L.rb, &L.dy, &L.r, &L.p, &L.q, &L.zz, &L.rhs, &L.sc};
-    for (double** pp : mv) {
-      *pp = wsp;
-      wsp += L.m;
-    }
+    __syncthreads();
+    const Lp& L = s_L;
     const int m = L.m, n = L.n;
     // Starting point: x mid-box (or 1), w = u - x, z = v = 1, y = 0.
     double nb = 0.0;
@@ -499,6 +541,8 @@ __global__ __launch_bounds__(kThreads) void batch_ipm_kernel(const BatchMatrix* 
     };
     int stalled = 0;  // consecutive iterations whose two PCG solves both hit the cap
     int last_pcg = 0;
+    bool fp64 = false;  // FP64 factorizations (global memory) after the FP32 solves stalled
+    int fp64_from = -1;
     double best = 1e300;
     int best_it = 0;
     for (int it = 0; it < o.max_iter; ++it) {
@@ -560,17 +604,27 @@ __global__ __launch_bounds__(kThreads) void batch_ipm_kernel(const BatchMatrix* 
         status = 6;
         break;
       }
-      // Stop early when the FP32-preconditioned solves have stalled (the
-      // caller finishes such LPs on a path with FP64 factorizations): the
-      // solves keep hitting their cap, or 30 iterations without progress.
+      // When the FP32-preconditioned solves stall (both hit their cap in an
+      // iteration, or 30 iterations pass without progress) the LP switches to
+      // FP64 factorizations; stalling again in FP64 (three iterations) ends
+      // it with status 6, which the caller finishes on the per-LP path.
       const double metric = fmax(pres, fmax(dres, gap));
       if (metric < best) {
         best = metric;
         best_it = it;
       }
-      if (stalled >= 3 || it - best_it > 30) {
-        status = 6;
-        break;
+      if ((fp64 ? stalled >= 3 : stalled >= 1) || it - best_it > 30) {
+        if (fp64) {
+          status = 6;
+          break;
+        }
+        // The FP32 factor no longer preconditions well enough: from here on
+        // this LP's normal matrix is factored in FP64 (global memory).
+        fp64 = true;
+        fp64_from = it;
+        stalled = 0;
+        best_it = it;
+        note(it, 15, -1.0);
       }
       const double mu = xz / fmax(nact + nb, 1.0);
       note(it, 4, mu);
@@ -587,13 +641,17 @@ __global__ __launch_bounds__(kThreads) void batch_ipm_kernel(const BatchMatrix* 
       __syncthreads();
       long long t1 = clock64();
       note(it, 9, static_cast<double>(t1 - t0));
-      assemble(L, M, 0.0);
+      if (fp64) assemble(L, L.M64, 0.0);
+      else assemble(L, M, 0.0);
       t0 = clock64();
       note(it, 10, static_cast<double>(t0 - t1));
-      note(it, 15, cholesky(L, M));
+      note(it, 15, fp64 ? cholesky(L, L.M64) : cholesky(L, M));
       t1 = clock64();
       note(it, 11, static_cast<double>(t1 - t0));
-      invert_lower(L, M, stage);
+      if (fp64) invert_lower(L, L.M64, stage);
+      else invert_lower(L, M, stage);
+      // inexact Newton while mu is large: CG relative residual max(pcg_tol, min(1e-6, 1e-3 mu))
+      const double ptol = fp64 ? o.pcg_tol : fmax(o.pcg_tol, fmin(1e-6, 1e-3 * mu));
       t0 = clock64();
       note(it, 12, static_cast<double>(t0 - t1));
       // predictor: r1 = -x z, r2 = -w v (in dxa/dwa scratch: use dza/dva as rhs storage)
@@ -609,7 +667,8 @@ __global__ __launch_bounds__(kThreads) void batch_ipm_kernel(const BatchMatrix* 
       }
       __syncthreads();
       {
-        const int k1 = newton(L, M, stage, red, L.dxa, L.dwa, L.dx, L.dw, L.dz, L.dv, o);
+        const int k1 = fp64 ? newton(L, L.M64, stage, red, L.dxa, L.dwa, L.dx, L.dw, L.dz, L.dv, ptol, o)
+                            : newton(L, M, stage, red, L.dxa, L.dwa, L.dx, L.dw, L.dz, L.dv, ptol, o);
         pcg_total += k1;
         last_pcg = k1;
         note(it, 7, k1);
@@ -633,7 +692,8 @@ __global__ __launch_bounds__(kThreads) void batch_ipm_kernel(const BatchMatrix* 
       }
       __syncthreads();
       {
-        const int k2 = newton(L, M, stage, red, L.dxa, L.dwa, L.dx, L.dw, L.dz, L.dv, o);
+        const int k2 = fp64 ? newton(L, L.M64, stage, red, L.dxa, L.dwa, L.dx, L.dw, L.dz, L.dv, ptol, o)
+                            : newton(L, M, stage, red, L.dxa, L.dwa, L.dx, L.dw, L.dz, L.dv, ptol, o);
         pcg_total += k2;
         note(it, 8, k2);
         stalled = (k2 >= o.pcg_max && last_pcg >= o.pcg_max) ? stalled + 1 : 0;
@@ -665,7 +725,7 @@ __global__ __launch_bounds__(kThreads) void batch_ipm_kernel(const BatchMatrix* 
       r.status = status;
       r.iterations = iters;
       r.pcg_iterations = pcg_total;
-      r.pad = 0;
+      r.fp64_from = fp64_from;
       r.objective = pobj;
       r.duality_gap = gap;
       r.primal_residual = pres;
@@ -775,7 +835,7 @@ int sov_batch_solve(sov_batch* h, int32_t count, const int32_t* matrix_id, const
     descs[k] = {id, 0, b_off[k], x_off[k], ws_total};
     b_total = std::max<long long>(b_total, b_off[k] + mt.m);
     x_total = std::max<long long>(x_total, x_off[k] + mt.n);
-    ws_total += 16LL * mt.n + 9LL * mt.m;
+    ws_total += 16LL * mt.n + 9LL * mt.m + static_cast<long long>(mt.m) * (mt.m + 1) / 2;
   }
   TmpBuffers tmp;  // per-solve device buffers, freed at the end of the call
   auto* d_mats = tmp.get<BatchMatrix>(h->mats.size());

@@ -20,6 +20,8 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <limits>
 #include <memory>
 #include <queue>
@@ -210,6 +212,11 @@ int finish_unconverged(const sov_lp_model& mdl, const std::vector<ChildSpec>& ki
       where.push_back(static_cast<int>(k));
     }
   if (redo.empty()) return SOV_OK;
+  if (const char* tr = std::getenv("SOVEREIGN_SB_TRACE"); tr && tr[0] == '1') {
+    std::fprintf(stderr, "[strong-branch] %zu of %zu children left unconverged by the batch:", redo.size(), out.size());
+    for (int k : where) std::fprintf(stderr, " %d(st%d,it%d)", k, out[k].status, out[k].iterations);
+    std::fprintf(stderr, "\n");
+  }
   std::vector<ChildOut> again;
   const int rc = solve_children_one_by_one(mdl, redo, F, opt, again, lp_solves);
   if (rc != SOV_OK) return rc;
