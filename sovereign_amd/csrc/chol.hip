@@ -36,17 +36,22 @@ constexpr int kPanelBuffers = 3;      // L21 buffers in rotation (see factor)
 // inverse X = L^-1 of its factor. The block lives in LDS as S (column-major,
 // leading dimension kPanelLd): L in the lower triangle, X^T in the strict
 // upper triangle (X(i, k) at S[i * kPanelLd + k] for i > k) and the diagonal
-// of X in xdiag. It is processed in sub-panels of kSub = 32 columns:
-//   A  wavefront 0 factors the 32 x 32 diagonal block in registers, with no
-//      workgroup barrier: lanes 0-31 each hold a row of the block, lanes
-//      32-63 each a column of its inverse, and both halves run the same
-//      right-looking update against the newly finished column of L, which
-//      is broadcast from the lanes that own it;
-//   B  rows below it inside the block: L(r, sub) = A(r, sub) inv(L_ss)^T;
-//   Y/X the inverse blocks left of the diagonal in this block row,
-//      X(t, <t) = -inv(L_tt) (L(t, <t) X(<t, <t));
-//   C  the rank-32 update of the trailing lower part of the block.
-// B, Y/X and C are 16 x 16 MFMA tiles with operands read from LDS.
+// of X in xdiag. It is processed in sub-panels of kSub = 32 columns; for
+// sub-panel c:
+//   A  its 32 x 32 diagonal block and that block's inverse, by wavefront 0
+//      alone (two 16 x 16 register leaves joined by MFMA products);
+//   B  the rows below it inside the block: L(r, c) = A(r, c) inv(L_cc)^T;
+//   C  the rank-32 update of the trailing lower part of the block;
+//   Y/X the inverse blocks left of its diagonal block,
+//      X(c, <c) = -inv(L_cc) Y,  Y = L(c, <c) X(<c, <c).
+// Look-ahead: once B(c) is stored, wavefront 0 applies C(c) to the next
+// sub-panel's diagonal block only and goes straight on to A(c + 1), while
+// wavefronts 1-7 apply the rest of C(c) and form X(c) and Y(c + 1). These
+// are grouped by 16-column group, one wavefront per group, so that Y(c + 1)
+// finds the X(c) it needs (the same columns) already written by its own
+// wavefront. The critical path per sub-panel is then A, B and the small
+// diagonal part of C. B, C, Y and X are 16 x 16 MFMA tiles with operands
+// read from LDS.
 // The pivot rule is the reference LDL^T's: a pivot <= 1e-13 x |its diagonal
 // before elimination| is dropped (L_jj = DropPivot), a non-finite or
 // < 1e-30 pivot is a breakdown. X goes to inv_out (leading dimension ldi,
@@ -98,65 +103,89 @@ __device__ __forceinline__ void wave_lds_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
 }
 
-// Cholesky factor L and inverse X = L^-1 of the h x h (h <= 16) block at D
-// (column-major, leading dimension kPanelLd) by one wavefront, in registers.
-// The 64 lanes form an 8 x 8 grid (p = lane & 7, q = lane >> 3); lane (p, q)
-// holds the elements (p + 8r, q + 8s), r, s < 2, of the block (f) and of X
-// (x). Step j scales column j of L and row j of X, then applies the rank-1
-// updates f -= L(:, j) L(:, j)^T below and right of j and x -= L(:, j) X(j, :)
-// below j; the factors of the update are fetched from the lanes that own
-// them and zeroed where the update does not apply, so the updates themselves
-// are unconditional. Writes L (lower), X (strict upper: X(i, k) at
-// D[i * kPanelLd + k]) and 1 / L(j, j) to xd[j]. thr (lane j): drop
-// threshold of column j. Pivot rule as described above the kernel.
+// 1 / sqrt(d) in the factor's precision: the hardware estimate refined by
+// Newton steps x <- x (1.5 - 0.5 d x^2), two in FP64, one in FP32.
 template <typename T>
-__device__ __forceinline__ void leaf_factor(T* D, int h, double thr, T* xd, FactorFlags* flags, int lane) {
+struct LeafMath;
+template <>
+struct LeafMath<double> {
+  __device__ static __forceinline__ double rsqrt(double d) {
+    double rs = __builtin_amdgcn_rsq(d);
+    rs = rs * (1.5 - 0.5 * d * rs * rs);
+    return rs * (1.5 - 0.5 * d * rs * rs);
+  }
+};
+template <>
+struct LeafMath<float> {
+  __device__ static __forceinline__ float rsqrt(float d) {
+    const float rs = __builtin_amdgcn_rsqf(d);
+    return rs * (1.5f - 0.5f * d * rs * rs);
+  }
+};
+
+// Cholesky factor L and inverse X = L^-1 of the h x h (h <= 16) block at D (column-major, leading dimension kPanelLd) by one
+// wavefront, in registers. The 64 lanes form an 8 x 8 grid (p = lane & 7,
+// q = lane >> 3); lane (p, q) holds the elements (p + 8r, q + 8s), r, s < 2,
+// of the block (f) and of X (x). Step j scales column j of L below the
+// diagonal and zeroes it on and above (the pivot goes to dg of the lane
+// holding (j, j)), scales row j of X, then applies the rank-1 updates
+// f -= L(:, j) L(:, j)^T and x -= L(:, j) X(j, :); their factors are fetched
+// from the lanes that own them, and as column j is zero on and above the
+// diagonal, the updates are unconditional. The pivot of step j + 1 is formed
+// as soon as column j is scaled, a(j+1, j+1) - L(j+1, j)^2 from two lane
+// reads, so that its reciprocal square root overlaps the rest of step j.
+// Pivot arithmetic is in T. Writes L (lower), X (strict upper: X(i, k) at
+// D[i * kPanelLd + k]) and 1 / L(j, j) to xd[j]; counts dropped pivots and
+// breakdowns into flags once at the end. thr (lane j): drop threshold of
+// column j. Pivot rule as described above the kernel.
+template <typename T>
+__device__ __forceinline__ void leaf_factor(T* D, int h, T thr, T* xd, FactorFlags* flags, int lane) {
   constexpr int NR = kLeaf / 8;
-  const int p = lane & 7, q = lane >> 3;
-  T f[NR][NR], x[NR][NR];
+  // The lane masks of the steps are formed from an opaque copy of the lane
+  // index, so that they are computed here and not hoisted out of the
+  // caller's sub-panel loop (where they would stay live and spill).
+  int ln = lane;
+  asm volatile("" : "+v"(ln));
+  const int p = ln & 7, q = ln >> 3;
+  T f[NR][NR], x[NR][NR], dg[NR];
 #pragma unroll
-  for (int r = 0; r < NR; ++r)
+  for (int r = 0; r < NR; ++r) {
+    dg[r] = T(0);
 #pragma unroll
     for (int s2 = 0; s2 < NR; ++s2) {
       const int i = p + 8 * r, k = q + 8 * s2;
       f[r][s2] = (i < h && k <= i) ? D[k * kPanelLd + i] : T(0);
       x[r][s2] = i == k ? T(1) : T(0);
     }
-  // The pivot of step j + 1 is formed as soon as column j is scaled,
-  // a(j+1, j+1) - L(j+1, j)^2 from two lane reads, so that its reciprocal
-  // square root overlaps the broadcasts and updates of step j.
-  double dnext = static_cast<double>(read_lane(f[0][0], 0));
+  }
+  T xdr = T(1);  // lane j: 1 / L(j, j)
+  int ndrop = 0;
+  bool anybad = false;
+  T dnext = read_lane(f[0][0], 0);
 #pragma unroll
   for (int j = 0; j < kLeaf; ++j) {
-    if (j < h) {
+    if (j < h) {  // (the guard also keeps the steps from being interleaved, which costs registers)
       const int jp = j & 7, js = j >> 3;
-      const double dj = dnext;
+      const T dj = dnext;
       const bool finite = isfinite(dj);
-      const bool drop = finite && dj <= read_lane(thr, j);
-      const bool bad = !drop && (!finite || dj < 1e-30);
-      // 1/sqrt(dj): hardware estimate and two Newton steps; sqrt = dj / sqrt(dj)
-      double rs = __builtin_amdgcn_rsq(dj);
-      rs = rs * (1.5 - 0.5 * dj * rs * rs);
-      rs = rs * (1.5 - 0.5 * dj * rs * rs);
-      const T piv = drop ? DropPivot<T>::value : bad ? T(1) : static_cast<T>(dj * rs);
-      const T rpiv = drop ? T(1) / DropPivot<T>::value : bad ? T(1) : static_cast<T>(rs);
-      if (lane == 0) {
-        if (drop) atomicAdd(&flags->replaced, 1);
-        if (bad) atomicOr(&flags->status, 1);
-        xd[j] = rpiv;
-      }
-      if (q == jp) {  // column j of L: the pivot, and l = a / pivot below it
+      const bool drop = finite & (dj <= read_lane(thr, j));
+      const bool bad = !drop & (!finite | (dj < T(1e-30)));
+      ndrop += drop ? 1 : 0;
+      anybad = anybad || bad;
+      const T rs = LeafMath<T>::rsqrt(dj);
+      const T piv = drop ? DropPivot<T>::value : bad ? T(1) : dj * rs;
+      const T rpiv = drop ? T(1) / DropPivot<T>::value : bad ? T(1) : rs;
+      xdr = ln == j ? rpiv : xdr;
+      if (ln == 9 * jp) dg[js] = piv;
+      if (q == jp) {  // column j of L: a / pivot below the diagonal, zero on and above it
 #pragma unroll
-        for (int r = 0; r < NR; ++r) {
-          const int i = p + 8 * r;
-          f[r][js] = i == j ? piv : (i > j ? f[r][js] * rpiv : f[r][js]);
-        }
+        for (int r = 0; r < NR; ++r) f[r][js] = p + 8 * r > j ? f[r][js] * rpiv : T(0);
       }
       if (j + 1 < h) {
         const int j1 = j + 1, j1p = j1 & 7, j1s = j1 >> 3;
-        const T l = read_lane(f[j1s][js], j1p + 8 * jp);      // L(j+1, j)
-        const T a = read_lane(f[j1s][j1s], 9 * j1p);          // a(j+1, j+1) before this step's update
-        dnext = static_cast<double>(a) - static_cast<double>(l) * static_cast<double>(l);
+        const T l = read_lane(f[j1s][js], j1p + 8 * jp);  // L(j+1, j)
+        const T a = read_lane(f[j1s][j1s], 9 * j1p);      // a(j+1, j+1) before this step's update
+        dnext = a - l * l;
       }
       if (p == jp) {  // row j of X
 #pragma unroll
@@ -164,15 +193,11 @@ __device__ __forceinline__ void leaf_factor(T* D, int h, double thr, T* xd, Fact
       }
       T lr[NR], lc[NR], xr[NR];
 #pragma unroll
-      for (int r = 0; r < NR; ++r) {
-        const T v = bcast_from(f[r][js], p + 8 * jp);  // L(p + 8r, j)
-        lr[r] = p + 8 * r > j ? v : T(0);
-      }
+      for (int r = 0; r < NR; ++r) lr[r] = bcast_from(f[r][js], p + 8 * jp);  // L(p + 8r, j)
 #pragma unroll
       for (int s2 = 0; s2 < NR; ++s2) {
-        const T v = bcast_from(f[s2][js], q + 8 * jp);  // L(q + 8 s2, j)
-        lc[s2] = q + 8 * s2 > j ? v : T(0);
-        xr[s2] = bcast_from(x[js][s2], jp + 8 * q);     // X(j, q + 8 s2)
+        lc[s2] = bcast_from(f[s2][js], q + 8 * jp);  // L(q + 8 s2, j)
+        xr[s2] = bcast_from(x[js][s2], jp + 8 * q);  // X(j, q + 8 s2)
       }
 #pragma unroll
       for (int r = 0; r < NR; ++r)
@@ -188,9 +213,208 @@ __device__ __forceinline__ void leaf_factor(T* D, int h, double thr, T* xd, Fact
 #pragma unroll
     for (int s2 = 0; s2 < NR; ++s2) {
       const int i = p + 8 * r, k = q + 8 * s2;
-      if (i < h && k <= i) D[k * kPanelLd + i] = f[r][s2];  // L(i, k)
-      if (i < h && k < i) D[i * kPanelLd + k] = x[r][s2];   // X(i, k), above the diagonal
+      if (i < h && k <= i) D[k * kPanelLd + i] = i == k ? dg[r] : f[r][s2];  // L(i, k)
+      if (i < h && k < i) D[i * kPanelLd + k] = x[r][s2];                    // X(i, k), above the diagonal
     }
+  if (ln < h) xd[ln] = xdr;
+  if (ln == 0) {
+    if (ndrop) atomicAdd(&flags->replaced, ndrop);
+    if (anybad) atomicOr(&flags->status, 1);
+  }
+}
+
+// Phase A for sub-panel c (width sb) by one wavefront: the two 16 x 16
+// leaves (leaf_factor) joined by three 16 x 16 x 16 MFMA products from LDS,
+// L21 = A21 X11^T, A22 -= L21 L21^T, X21 = -X22 (L21 X11). Ys: a 16 x 16
+// scratch only this wavefront uses.
+template <typename T>
+__device__ __forceinline__ void panel_diag(T* S, int c, int sb, T* xdiag, const double* d0, T* Ys,
+                                           FactorFlags* flags, int lane) {
+  typedef typename Mfma<T>::acc_t acc_t;
+  const int li = lane & 15, lk = lane >> 4;
+  T* D = S + c * (kPanelLd + 1);  // element (i, k) of the diagonal block at D[k * kPanelLd + i]
+  const int h1 = min(kLeaf, sb), h2 = sb - h1;
+  const T thr1 = static_cast<T>(lane < h1 ? 1e-13 * fabs(d0[c + lane]) : 0.0);
+  leaf_factor<T>(D, h1, thr1, xdiag + c, flags, lane);
+  if (h2 <= 0) return;
+  wave_lds_sync();
+  {  // L21 = A21 X11^T over rows kLeaf.., columns 0..kLeaf-1
+    T av[kLeaf / 4], bv[kLeaf / 4];
+    const T xl = xdiag[c + li];
+#pragma unroll
+    for (int s = 0; s < kLeaf / 4; ++s) {
+      const int kk = 4 * s + lk;
+      const T a0 = D[kk * kPanelLd + kLeaf + li], b0 = D[li * kPanelLd + kk];
+      av[s] = li < h2 ? a0 : T(0);
+      bv[s] = li > kk ? b0 : (li == kk ? xl : T(0));
+    }
+    __builtin_amdgcn_sched_barrier(0);
+    acc_t acc = acc_t{0, 0, 0, 0};
+#pragma unroll
+    for (int s = 0; s < kLeaf / 4; ++s) acc = Mfma<T>::step(av[s], bv[s], acc);
+#pragma unroll
+    for (int g = 0; g < 4; ++g) {
+      const int i = Mfma<T>::row(lk, g);
+      if (i < h2) D[li * kPanelLd + kLeaf + i] = acc[g];
+    }
+  }
+  wave_lds_sync();
+  {  // A22 -= L21 L21^T (lower part) and Y = L21 X11
+    T lv[kLeaf / 4], xv[kLeaf / 4];
+    const T xl = xdiag[c + li];
+#pragma unroll
+    for (int s = 0; s < kLeaf / 4; ++s) {
+      const int kk = 4 * s + lk;
+      const T l0 = D[kk * kPanelLd + kLeaf + li], x0 = D[kk * kPanelLd + li];
+      lv[s] = li < h2 ? l0 : T(0);                    // L21(li, kk)
+      xv[s] = kk > li ? x0 : (kk == li ? xl : T(0));  // X11(kk, li)
+    }
+    __builtin_amdgcn_sched_barrier(0);
+    acc_t acc = acc_t{0, 0, 0, 0}, yacc = acc_t{0, 0, 0, 0};
+#pragma unroll
+    for (int s = 0; s < kLeaf / 4; ++s) {
+      acc = Mfma<T>::step(lv[s], lv[s], acc);
+      yacc = Mfma<T>::step(lv[s], xv[s], yacc);
+    }
+#pragma unroll
+    for (int g = 0; g < 4; ++g) {
+      const int i = Mfma<T>::row(lk, g);
+      if (i < h2 && i >= li) D[(kLeaf + li) * kPanelLd + kLeaf + i] -= acc[g];
+      Ys[i * kLeafLd + li] = yacc[g];
+    }
+  }
+  wave_lds_sync();
+  T* D2 = D + kLeaf * (kPanelLd + 1);
+  const T thr2 = static_cast<T>(lane < h2 ? 1e-13 * fabs(d0[c + kLeaf + lane]) : 0.0);
+  leaf_factor<T>(D2, h2, thr2, xdiag + c + kLeaf, flags, lane);
+  wave_lds_sync();
+  {  // X21 = -X22 Y
+    T av[kLeaf / 4], bv[kLeaf / 4];
+    const T xl = xdiag[c + kLeaf + li];
+#pragma unroll
+    for (int s = 0; s < kLeaf / 4; ++s) {
+      const int kk = 4 * s + lk;
+      const T a0 = D2[li * kPanelLd + kk];
+      av[s] = li > kk ? a0 : (li == kk ? xl : T(0));
+      bv[s] = Ys[kk * kLeafLd + li];
+    }
+    __builtin_amdgcn_sched_barrier(0);
+    acc_t acc = acc_t{0, 0, 0, 0};
+#pragma unroll
+    for (int s = 0; s < kLeaf / 4; ++s) acc = Mfma<T>::step(av[s], bv[s], acc);
+#pragma unroll
+    for (int g = 0; g < 4; ++g) {
+      const int i = Mfma<T>::row(lk, g);
+      if (i < h2) D[(kLeaf + i) * kPanelLd + li] = -acc[g];
+    }
+  }
+}
+
+// In the tile helpers below, each tile's operands are read from LDS at
+// in-range addresses without branching (values outside the tile's triangle
+// or rows are selected to zero afterwards), all before the tile's MFMA
+// chain, so that the reads overlap instead of each waiting in front of its
+// instruction.
+
+// X tile of sub-panel c (width sb): rows 16 half.. of the sub-panel, the 16
+// columns of group g (< c / 16): X(c + i, x) = -sum_v inv(L_cc)(i, v) Y(v, x),
+// with Y(v, x) at Yt[x * kSubLd + v].
+template <typename T>
+__device__ __forceinline__ void panel_x_tile(T* S, const T* Yt, const T* xdiag, int c, int sb, int half, int g,
+                                             int li, int lk) {
+  typedef typename Mfma<T>::acc_t acc_t;
+  const T* D = S + c * (kPanelLd + 1);
+  const int i = 16 * half + li;  // row of inv(L_cc) of this lane
+  const int x0 = 16 * g;
+  const T xi = xdiag[c + i];
+  T av[kSub / 4], bv[kSub / 4];
+#pragma unroll
+  for (int s2 = 0; s2 < kSub / 4; ++s2) {
+    const int v = 4 * s2 + lk;
+    const T a0 = D[i * kPanelLd + v];
+    av[s2] = i < sb ? (v < i ? a0 : (v == i ? xi : T(0))) : T(0);
+    bv[s2] = Yt[(x0 + li) * kSubLd + v];
+  }
+  __builtin_amdgcn_sched_barrier(0);
+  acc_t acc = acc_t{0, 0, 0, 0};
+#pragma unroll
+  for (int s2 = 0; s2 < kSub / 4; ++s2) acc = Mfma<T>::step(av[s2], bv[s2], acc);
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const int ii = 16 * half + Mfma<T>::row(lk, q);
+    if (ii < sb) S[(c + ii) * kPanelLd + x0 + li] = -acc[q];
+  }
+}
+
+// Y tile of sub-panel c (width sb): rows 16 half.. of the sub-panel, the 16
+// columns x of group g (x0 = 16 g < c):
+// Y(v, x) = sum_{x0 <= j < c} L(c + v, j) X(j, x) into Yt[x * kSubLd + v]
+// (X(j, x) = 0 for j < x0; blocks of 16 j's, the next block's operands read
+// while the current one's MFMAs run).
+template <typename T>
+__device__ __forceinline__ void panel_y_tile(const T* S, T* Yt, const T* xdiag, int c, int sb, int half, int g,
+                                             int li, int lk) {
+  typedef typename Mfma<T>::acc_t acc_t;
+  const int v = 16 * half + li;  // row of L(c + v, .) of this lane
+  const int x0 = 16 * g;
+  const int kx = x0 + li;        // column of X of this lane
+  const T xk = xdiag[kx];
+  auto read_block = [&](int j0, T* ao, T* bo) {
+#pragma unroll
+    for (int s2 = 0; s2 < 4; ++s2) {
+      const int jj = j0 + 4 * s2 + lk;
+      const T a0 = S[jj * kPanelLd + c + v], b0 = S[jj * kPanelLd + kx];
+      ao[s2] = v < sb ? a0 : T(0);
+      bo[s2] = jj > kx ? b0 : (jj == kx ? xk : T(0));
+    }
+  };
+  acc_t acc = acc_t{0, 0, 0, 0};
+  T av[4], bv[4];
+  read_block(x0, av, bv);
+  for (int j0 = x0; j0 < c; j0 += 16) {
+    T an[4] = {T(0), T(0), T(0), T(0)}, bn[4] = {T(0), T(0), T(0), T(0)};
+    if (j0 + 16 < c) read_block(j0 + 16, an, bn);
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int s2 = 0; s2 < 4; ++s2) acc = Mfma<T>::step(av[s2], bv[s2], acc);
+#pragma unroll
+    for (int s2 = 0; s2 < 4; ++s2) {
+      av[s2] = an[s2];
+      bv[s2] = bn[s2];
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < 4; ++q) Yt[(x0 + li) * kSubLd + 16 * half + Mfma<T>::row(lk, q)] = acc[q];
+}
+
+// Lower tile t (row-major order over the lower triangle of 16 x 16 tiles) of
+// the rank-32 update by sub-panel c (width kSub) of the block's trailing part
+// (rows and columns from c + kSub, below nb).
+template <typename T>
+__device__ __forceinline__ void panel_c_tile(T* S, int c, int nb, int t, int li, int lk) {
+  typedef typename Mfma<T>::acc_t acc_t;
+  int ti = 0;
+  while ((ti + 1) * (ti + 2) / 2 <= t) ++ti;
+  const int tj = t - ti * (ti + 1) / 2;
+  const int r = c + kSub + 16 * ti + li, q = c + kSub + 16 * tj + li;
+  const int rr = r < nb ? r : 0, qq = q < nb ? q : 0;
+  T av[kSub / 4], bv[kSub / 4];
+#pragma unroll
+  for (int s2 = 0; s2 < kSub / 4; ++s2) {
+    const T* col = S + (c + 4 * s2 + lk) * kPanelLd;
+    const T a0 = col[rr], b0 = col[qq];
+    av[s2] = r < nb ? a0 : T(0);
+    bv[s2] = q < nb ? b0 : T(0);
+  }
+  __builtin_amdgcn_sched_barrier(0);
+  acc_t acc = acc_t{0, 0, 0, 0};
+#pragma unroll
+  for (int s2 = 0; s2 < kSub / 4; ++s2) acc = Mfma<T>::step(av[s2], bv[s2], acc);
+#pragma unroll
+  for (int g = 0; g < 4; ++g) {
+    const int rr2 = c + kSub + 16 * ti + Mfma<T>::row(lk, g);
+    if (rr2 < nb && q < nb && rr2 >= q) S[q * kPanelLd + rr2] -= acc[g];
+  }
 }
 
 template <typename T>
@@ -201,9 +425,10 @@ __global__ __launch_bounds__(kPanelThreads) void potrf_panel_kernel(T* __restric
   typedef typename Mfma<T>::acc_t acc_t;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   T* S = reinterpret_cast<T*>(smem_raw);                   // [kPanel][kPanelLd]
-  T* Yt = S + kPanel * kPanelLd;                           // [kPanel - kSub][kSubLd], Y(v, kk) at kk * kSubLd + v
+  T* Yt = S + kPanel * kPanelLd;                           // [kPanel - kSub][kSubLd], Y(v, x) at x * kSubLd + v
   T* xdiag = Yt + (kPanel - kSub) * kSubLd;                // [kPanel]
-  double* d0 = reinterpret_cast<double*>(xdiag + kPanel);  // [kPanel]
+  T* Ys = xdiag + kPanel;                                  // [kLeaf][kLeafLd], wavefront 0's scratch
+  double* d0 = reinterpret_cast<double*>(Ys + kLeaf * kLeafLd);  // [kPanel]
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int li = lane & 15, lk = lane >> 4;  // MFMA operand row / k lane group
   constexpr int kWaves = kPanelThreads / 64;
@@ -221,177 +446,90 @@ __global__ __launch_bounds__(kPanelThreads) void potrf_panel_kernel(T* __restric
   }
   __syncthreads();
 
-  for (int c = 0; c < nb; c += kSub) {
-    const int sb = min(kSub, nb - c);  // width of this sub-panel
-    const int rb = nb - c - sb;        // rows below it (sb == kSub when rb > 0)
-    T* D = S + c * (kPanelLd + 1);     // its diagonal block: element (i, k) at D[k * kPanelLd + i]
-
-    // ---- A: the sb x sb diagonal block and its inverse, by wavefront 0
-    // alone (no workgroup barrier): two 16 x 16 leaves factored in
-    // registers (leaf_factor) and joined by three 16 x 16 x 16 MFMA products
-    // from LDS: L21 = A21 X11^T, A22 -= L21 L21^T, X21 = -X22 (L21 X11).
-    if (wave == 0) {
-      const int h1 = min(kLeaf, sb), h2 = sb - h1;
-      leaf_factor<T>(D, h1, lane < h1 ? 1e-13 * fabs(d0[c + lane]) : 0.0, xdiag + c, flags, lane);
-      if (h2 > 0) {
-        wave_lds_sync();
-        T* Ys = Yt;  // 16 x 16 scratch, Y(i, k) at Ys[i * kLeafLd + k]
-        {  // L21 = A21 X11^T over rows kLeaf.., columns 0..kLeaf-1
-          acc_t acc = acc_t{0, 0, 0, 0};
+  // Iteration c: B and the rest of C for sub-panel c (none for c = -kSub),
+  // and the diagonal block of sub-panel c + kSub, so that the diagonal-block
+  // code exists once.
+  int c = -kSub;
+  for (;;) {
+    const int cn = c + kSub, sbn = min(kSub, nb - cn);
+    int nct = 0;
+    if (c >= 0) {
+      const T* D = S + c * (kPanelLd + 1);  // sub-panel c's diagonal block (sb = kSub: rows follow)
+      const int rb = nb - cn;               // rows below it
+      // ---- B: L(r, c) = A(r, c) inv(L_cc)^T, 16 x 16 tiles (computed, then
+      // stored after a barrier: the operands are read in place)
+      const int nrt = (rb + 15) >> 4;  // 16-row tiles below the sub-panel
+      const int nbt = 2 * nrt;         // B tiles (two 16-column tiles each row; <= 12)
+      nct = nrt * (nrt + 1) / 2;
+      acc_t bacc[2];
 #pragma unroll
-          for (int k = 0; k < kLeaf; k += 4) {
-            const int kk = k + lk;
-            const T av = li < h2 ? D[kk * kPanelLd + kLeaf + li] : T(0);
-            const T bv = li > kk ? D[li * kPanelLd + kk] : (li == kk ? xdiag[c + li] : T(0));
-            acc = Mfma<T>::step(av, bv, acc);
-          }
+      for (int n = 0; n < 2; ++n) {
+        const int t = wave + n * kWaves;
+        if (t >= nbt) break;
+        const int r = cn + 16 * (t >> 1) + li;  // A row of this lane
+        const int q = 16 * (t & 1) + li;        // B column of this lane
+        const int rr = r < nb ? r : 0;
+        const T xq = xdiag[c + q];
+        T av[kSub / 4], bv[kSub / 4];
 #pragma unroll
-          for (int g = 0; g < 4; ++g) {
-            const int i = Mfma<T>::row(lk, g);
-            if (i < h2) D[li * kPanelLd + kLeaf + i] = acc[g];
-          }
+        for (int s2 = 0; s2 < kSub / 4; ++s2) {
+          const int kk = 4 * s2 + lk;
+          const T a0 = S[(c + kk) * kPanelLd + rr], b0 = D[q * kPanelLd + kk];
+          av[s2] = r < nb ? a0 : T(0);
+          bv[s2] = kk < q ? b0 : (kk == q ? xq : T(0));
         }
-        wave_lds_sync();
-        {  // A22 -= L21 L21^T (lower part) and Y = L21 X11
-          acc_t acc = acc_t{0, 0, 0, 0}, yacc = acc_t{0, 0, 0, 0};
+        __builtin_amdgcn_sched_barrier(0);
+        acc_t acc = acc_t{0, 0, 0, 0};
 #pragma unroll
-          for (int k = 0; k < kLeaf; k += 4) {
-            const int kk = k + lk;
-            const T lv = li < h2 ? D[kk * kPanelLd + kLeaf + li] : T(0);  // L21(li, kk)
-            acc = Mfma<T>::step(lv, lv, acc);
-            const T xv = kk > li ? D[kk * kPanelLd + li] : (kk == li ? xdiag[c + li] : T(0));  // X11(kk, li)
-            yacc = Mfma<T>::step(lv, xv, yacc);
-          }
-#pragma unroll
-          for (int g = 0; g < 4; ++g) {
-            const int i = Mfma<T>::row(lk, g);
-            if (i < h2 && i >= li) D[(kLeaf + li) * kPanelLd + kLeaf + i] -= acc[g];
-            Ys[i * kLeafLd + li] = yacc[g];
-          }
-        }
-        wave_lds_sync();
-        T* D2 = D + kLeaf * (kPanelLd + 1);
-        leaf_factor<T>(D2, h2, lane < h2 ? 1e-13 * fabs(d0[c + kLeaf + lane]) : 0.0, xdiag + c + kLeaf, flags,
-                       lane);
-        wave_lds_sync();
-        {  // X21 = -X22 Y
-          acc_t acc = acc_t{0, 0, 0, 0};
-#pragma unroll
-          for (int k = 0; k < kLeaf; k += 4) {
-            const int kk = k + lk;
-            const T av = li > kk ? D2[li * kPanelLd + kk] : (li == kk ? xdiag[c + kLeaf + li] : T(0));
-            const T bv = Ys[kk * kLeafLd + li];
-            acc = Mfma<T>::step(av, bv, acc);
-          }
-#pragma unroll
-          for (int g = 0; g < 4; ++g) {
-            const int i = Mfma<T>::row(lk, g);
-            if (i < h2) D[(kLeaf + i) * kPanelLd + li] = -acc[g];
-          }
-        }
-      }
-    }
-    __syncthreads();
-
-    // ---- B (compute): L(r, sub) = A(r, sub) inv(L_ss)^T as 16 x 16 MFMA
-    // tiles; Y = L(t, <t) X(<t, <t) into Yt.
-    const int nrt = (rb + 15) >> 4;  // 16-row tiles below the sub-panel
-    const int nbt = 2 * nrt;         // B tiles (two 16-column tiles each row)
-    const int nyt = 2 * (c >> 4);    // Y tiles
-    acc_t bacc[3];
-#pragma unroll
-    for (int n = 0; n < 3; ++n) {  // tiles t = wave + n kWaves (nbt + nyt <= 24)
-      const int t = wave + n * kWaves;
-      if (t >= nbt + nyt) break;
-      acc_t acc = acc_t{0, 0, 0, 0};
-      if (t < nbt) {
-        const int r = c + sb + 16 * (t >> 1) + li;  // A row of this lane
-        const int q = 16 * (t & 1) + li;            // B column of this lane
-#pragma unroll 4
-        for (int k = 0; k < kSub; k += 4) {
-          const int kk = k + lk;
-          const T av = r < nb ? S[(c + kk) * kPanelLd + r] : T(0);
-          const T bv = kk < q ? D[q * kPanelLd + kk] : (kk == q ? xdiag[c + q] : T(0));
-          acc = Mfma<T>::step(av, bv, acc);
-        }
+        for (int s2 = 0; s2 < kSub / 4; ++s2) acc = Mfma<T>::step(av[s2], bv[s2], acc);
         bacc[n] = acc;
-      } else {
-        const int u = t - nbt;
-        const int v = 16 * (u & 1) + li;  // A row (block row t of L)
-        const int x0 = 16 * (u >> 1);     // first output column
-        const int kx = x0 + li;           // B column of this lane
-        for (int j0 = x0; j0 < c; j0 += 16) {  // X(j, kx) = 0 for j < x0
-#pragma unroll
-          for (int j = j0; j < j0 + 16; j += 4) {
-            const int jj = j + lk;
-            const T av = v < sb ? S[jj * kPanelLd + c + v] : T(0);
-            const T bv = jj > kx ? S[jj * kPanelLd + kx] : (jj == kx ? xdiag[kx] : T(0));
-            acc = Mfma<T>::step(av, bv, acc);
-          }
-        }
-#pragma unroll
-        for (int g = 0; g < 4; ++g) Yt[(x0 + li) * kSubLd + 16 * (u & 1) + Mfma<T>::row(lk, g)] = acc[g];
       }
-    }
-    __syncthreads();
-
-    // ---- B (store); X(t, <t) = -inv(L_tt) Y into the upper triangle ----
+      __syncthreads();
 #pragma unroll
-    for (int n = 0; n < 3; ++n) {
-      const int t = wave + n * kWaves;
-      if (t >= nbt + nyt) break;
-      if (t < nbt) {
-        const acc_t acc = bacc[n];
+      for (int n = 0; n < 2; ++n) {
+        const int t = wave + n * kWaves;
+        if (t >= nbt) break;
         const int q = 16 * (t & 1) + li;
 #pragma unroll
         for (int g = 0; g < 4; ++g) {
-          const int r = c + sb + 16 * (t >> 1) + Mfma<T>::row(lk, g);
-          if (r < nb) S[(c + q) * kPanelLd + r] = acc[g];
-        }
-      } else {
-        const int u = t - nbt;
-        const int i = 16 * (u & 1) + li;  // A row: inv(L_tt) row
-        const int x0 = 16 * (u >> 1);
-        acc_t acc = acc_t{0, 0, 0, 0};
-#pragma unroll 4
-        for (int vv = 0; vv < kSub; vv += 4) {
-          const int v = vv + lk;
-          const T av = i < sb ? (v < i ? D[i * kPanelLd + v] : (v == i ? xdiag[c + i] : T(0))) : T(0);
-          const T bv = Yt[(x0 + li) * kSubLd + v];
-          acc = Mfma<T>::step(av, bv, acc);
-        }
-#pragma unroll
-        for (int g = 0; g < 4; ++g) {
-          const int ii = 16 * (u & 1) + Mfma<T>::row(lk, g);
-          if (ii < sb) S[(c + ii) * kPanelLd + x0 + li] = -acc[g];
+          const int r = cn + 16 * (t >> 1) + Mfma<T>::row(lk, g);
+          if (r < nb) S[(c + q) * kPanelLd + r] = bacc[n][g];
         }
       }
+      __syncthreads();
     }
-    __syncthreads();
 
-    // ---- C: rank-32 update of the trailing lower block (lower tiles) ----
-    const int nct = nrt * (nrt + 1) / 2;
-    for (int t = wave; t < nct; t += kWaves) {
-      int ti = 0;
-      while ((ti + 1) * (ti + 2) / 2 <= t) ++ti;
-      const int tj = t - ti * (ti + 1) / 2;
-      const int r = c + sb + 16 * ti + li, q = c + sb + 16 * tj + li;
-      acc_t acc = acc_t{0, 0, 0, 0};
-#pragma unroll 4
-      for (int k = 0; k < kSub; k += 4) {
-        const T* col = S + (c + k + lk) * kPanelLd;
-        const T av = r < nb ? col[r] : T(0);
-        const T bv = q < nb ? col[q] : T(0);
-        acc = Mfma<T>::step(av, bv, acc);
+    // ---- look-ahead (see above the kernel): wavefront 0 updates the next
+    // diagonal block (the C tiles of its two tile rows) and factors it; the
+    // helper wavefronts take the column groups (X(c), then Y(c + 1)) and the
+    // remaining C tiles. Wavefront 4, which shares wavefront 0's SIMD, is not
+    // a helper, so that the latency-bound factorization keeps its SIMD.
+    const int ndiag = min(nct, 3);
+    if (wave == 0) {
+      for (int t = 0; t < ndiag; ++t) panel_c_tile<T>(S, c, nb, t, li, lk);
+      wave_lds_sync();
+      panel_diag<T>(S, cn, sbn, xdiag, d0, Ys, flags, lane);
+    } else if (c >= 0 && wave != kWaves / 2) {
+      constexpr int kHelpers = kWaves - 2;
+      const int hw = wave - 1 - (wave > kWaves / 2 ? 1 : 0);  // 0 .. kHelpers - 1
+      for (int g = hw; g < (cn >> 4); g += kHelpers) {
+        if (g < (c >> 4)) {
+          panel_x_tile<T>(S, Yt, xdiag, c, kSub, 0, g, li, lk);
+          panel_x_tile<T>(S, Yt, xdiag, c, kSub, 1, g, li, lk);
+          wave_lds_sync();
+        }
+        panel_y_tile<T>(S, Yt, xdiag, cn, sbn, 0, g, li, lk);
+        panel_y_tile<T>(S, Yt, xdiag, cn, sbn, 1, g, li, lk);
       }
-#pragma unroll
-      for (int g = 0; g < 4; ++g) {
-        const int rr = c + sb + 16 * ti + Mfma<T>::row(lk, g);
-        if (rr < nb && q < nb && rr >= q) S[q * kPanelLd + rr] -= acc[g];
-      }
+      for (int t = ndiag + hw; t < nct; t += kHelpers) panel_c_tile<T>(S, c, nb, t, li, lk);
     }
     __syncthreads();
+    c = cn;
+    if (c + kSub >= nb) break;  // no rows below sub-panel c
   }
+  // X of the last sub-panel (its Y was formed in the last look-ahead)
+  for (int u = wave; u < 2 * (c >> 4); u += kWaves) panel_x_tile<T>(S, Yt, xdiag, c, nb - c, u & 1, u >> 1, li, lk);
+  __syncthreads();
 
   // X to the inverse buffer, zero outside the nb x nb lower triangle. (L11
   // itself is not stored back: the solves and the panel TRSM use only X.)
@@ -427,7 +565,8 @@ __global__ void reset_flags_kernel(FactorFlags* f) {
 
 template <typename T>
 constexpr std::size_t panel_lds_bytes() {
-  return sizeof(T) * (kPanel * kPanelLd + (kPanel - kSub) * kSubLd + kPanel) + sizeof(double) * kPanel;
+  return sizeof(T) * (kPanel * kPanelLd + (kPanel - kSub) * kSubLd + kPanel + kLeaf * kLeafLd) +
+         sizeof(double) * kPanel;
 }
 
 template <typename T>

@@ -112,25 +112,59 @@ __global__ __launch_bounds__(256) void gemm_update_kernel(int M, int N, int K, c
 #pragma unroll
     for (int j = 0; j < JN; ++j) acc[i][j] = acc_t{0, 0, 0, 0};
 
-  // Global -> register staging of one chunk of kChunk k-columns.
+  // Offset of C element (i, j, g) of this lane (see the epilogue), or -1
+  // where the element is outside C or above the diagonal of a LOWER tile.
+  auto c_offset = [&](int i, int j, int g) -> long long {
+    const int gr = r0 + wr + 16 * i + lrow;
+    const int gc = c0 + wc + 16 * j + Mfma<T>::row(lk, g);
+    const bool in = gr < M && gc < N && (!LOWER || gr >= gc);
+    return in ? gr + static_cast<long long>(gc) * ldc : -1;
+  };
+  // The old values of C for an update (!STORE): every load in flight at
+  // once, out-of-range lanes reading C[0] instead of branching. When they
+  // fit in 32 registers (64 x 64 tiles) they are loaded before the K loop,
+  // so that their latency overlaps the products; otherwise after it.
+  T cold[IM][JN][4];
+  auto load_c = [&]() {
+#pragma unroll
+    for (int i = 0; i < IM; ++i)
+#pragma unroll
+      for (int j = 0; j < JN; ++j)
+#pragma unroll
+        for (int g = 0; g < 4; ++g) {
+          const long long o = c_offset(i, j, g);
+          cold[i][j][g] = C[o < 0 ? 0 : o];
+        }
+  };
+  constexpr bool kEarlyC = !STORE && IM * JN * 4 * sizeof(T) <= 128;
+  if constexpr (kEarlyC) load_c();
+
+  // Global -> register staging of one chunk of kChunk k-columns
+  // (out-of-range elements read element 0 and are replaced by zero).
   T ra[QA], rb[QB];
   auto load_chunk = [&](int k0) {
 #pragma unroll
     for (int q = 0; q < QA; ++q) {
       const int r = tid % BM, k = tid / BM + (256 / BM) * q;
       const int gr = r0 + r, gk = k0 + k;
-      ra[q] = (gr < M && gk < K) ? A[gr + static_cast<long long>(gk) * lda] : T(0);
+      const bool in = gr < M && gk < K;
+      const T v = A[in ? gr + static_cast<long long>(gk) * lda : 0];
+      ra[q] = in ? v : T(0);
     }
 #pragma unroll
     for (int q = 0; q < QB; ++q) {
       if (NN) {
         const int k = tid % kChunk, c = tid / kChunk + (256 / kChunk) * q;
         const int gc = c0 + c, gk = k0 + k;
-        rb[q] = (gc < N && gk < K) ? B[gk + static_cast<long long>(gc) * ldb] : T(0);
+        const bool in = gc < N && gk < K;
+        const T v = B[in ? gk + static_cast<long long>(gc) * ldb : 0];
+        rb[q] = in ? v : T(0);
       } else {
         const int c = tid % BN, k = tid / BN + (256 / BN) * q;
         const int gc = c0 + c, gk = k0 + k;
-        rb[q] = (gc < N && gk < K) ? B[gc + static_cast<long long>(gk) * ldb] : T(0);
+        const bool in = gc < N && gk < K;
+        const T v = B[in ? gc + static_cast<long long>(gk) * ldb : 0];
+        rb[q] = in ? v : T(0);
       }
     }
   };
@@ -150,20 +184,28 @@ __global__ __launch_bounds__(256) void gemm_update_kernel(int M, int N, int K, c
     store_chunk();
     __syncthreads();
     if (k0 + kChunk < K) load_chunk(k0 + kChunk);
+    // Operands of kGroup k-steps are read from LDS together, so their
+    // latencies overlap, then the group's MFMAs run.
+    constexpr int kGroup = (IM + JN) * sizeof(T) <= 16 ? 8 : 4;
 #pragma unroll
-    for (int kk = 0; kk < kChunk; kk += 4) {
-      const int ka = (kk + lk) * LA, kb = (kk + lk) * LB;
-      T av[IM], bv[JN];
+    for (int kg = 0; kg < kChunk; kg += 4 * kGroup) {
+      T av[kGroup][IM], bv[kGroup][JN];
 #pragma unroll
-      for (int i = 0; i < IM; ++i) av[i] = As[ka + wr + 16 * i + lrow];
+      for (int s = 0; s < kGroup; ++s) {
+        const int ka = (kg + 4 * s + lk) * LA, kb = (kg + 4 * s + lk) * LB;
 #pragma unroll
-      for (int j = 0; j < JN; ++j) bv[j] = Bs[kb + wc + 16 * j + lrow];
+        for (int i = 0; i < IM; ++i) av[s][i] = As[ka + wr + 16 * i + lrow];
+#pragma unroll
+        for (int j = 0; j < JN; ++j) bv[s][j] = Bs[kb + wc + 16 * j + lrow];
+      }
       // The B tile goes in as the instruction's first operand, so each
       // 16 x 16 result comes out transposed: a lane holds one row of C.
 #pragma unroll
-      for (int i = 0; i < IM; ++i)
+      for (int s = 0; s < kGroup; ++s)
 #pragma unroll
-        for (int j = 0; j < JN; ++j) acc[i][j] = Mfma<T>::step(bv[j], av[i], acc[i][j]);
+        for (int i = 0; i < IM; ++i)
+#pragma unroll
+          for (int j = 0; j < JN; ++j) acc[i][j] = Mfma<T>::step(bv[s][j], av[s][i], acc[i][j]);
     }
   }
 
@@ -173,21 +215,18 @@ __global__ __launch_bounds__(256) void gemm_update_kernel(int M, int N, int K, c
   // A-tile rows (C rows), so lane l holds C row (l & 15) of the block for
   // four C columns: one store instruction writes 16 consecutive rows of each
   // of four columns.
+  if constexpr (!STORE && !kEarlyC) load_c();
 #pragma unroll
-  for (int i = 0; i < IM; ++i) {
-    const int gr = r0 + wr + 16 * i + lrow;
-    if (gr >= M) continue;
+  for (int i = 0; i < IM; ++i)
 #pragma unroll
-    for (int j = 0; j < JN; ++j) {
+    for (int j = 0; j < JN; ++j)
 #pragma unroll
       for (int g = 0; g < 4; ++g) {
-        const int gc = c0 + wc + 16 * j + Mfma<T>::row(lk, g);
-        if (gc >= N || (LOWER && gr < gc)) continue;
-        if (STORE) C[gr + static_cast<long long>(gc) * ldc] = alpha * acc[i][j][g];
-        else C[gr + static_cast<long long>(gc) * ldc] += alpha * acc[i][j][g];
+        const long long o = c_offset(i, j, g);
+        if (o < 0) continue;
+        if constexpr (STORE) C[o] = alpha * acc[i][j][g];
+        else C[o] = cold[i][j][g] + alpha * acc[i][j][g];
       }
-    }
-  }
 }
 
 // Host launcher: one product, or ny x nz of them offset by `bt`, with BM x BN
