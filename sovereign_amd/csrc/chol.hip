@@ -608,8 +608,10 @@ EventPool& pool() {
   return p;
 }
 
-// Opens a timed pair on `s` when timing is on (nullptr otherwise).
-TimedPair* timed_begin(hipStream_t s, int kind) {
+// A pair for one timed kernel launch whose own dispatch records both events
+// (hipExtLaunchKernel's start and stop events: no markers around it), when
+// timing is on for `kind` (nullptr otherwise).
+TimedPair* timed_kernel(int kind) {
   const int level = g_timing.load();
   if (level == 0 || (level == 2 && kind != kTimeSyrk && kind != kTimeSyrk32)) return nullptr;
   TimedPair* p = pool().next();
@@ -617,7 +619,13 @@ TimedPair* timed_begin(hipStream_t s, int kind) {
   p->kind = kind;
   p->flops = 0.0;
   p->launches = 0;
-  (void)hipEventRecord(p->a, s);
+  return p;
+}
+
+// Opens a timed pair on `s` when timing is on (nullptr otherwise).
+TimedPair* timed_begin(hipStream_t s, int kind) {
+  TimedPair* p = timed_kernel(kind);
+  if (p) (void)hipEventRecord(p->a, s);
   return p;
 }
 void timed_end(TimedPair* p, hipStream_t s, int launches, double flops) {
@@ -689,6 +697,7 @@ int DenseCholesky<T>::reserve(int m) {
   panel_ = sup_inv_ = tmp_ = nullptr;
   work_ = nullptr;
   flags_ = nullptr;
+  solve_graphs_.clear();  // they hold the old buffers' addresses
   SOV_HIP_TRY(hipMalloc(&panel_, sizeof(T) * static_cast<std::size_t>(m) * kPanel * kPanelBuffers));
   const std::size_t nsb = static_cast<std::size_t>((m + kSuper - 1) / kSuper);
   const std::size_t sq = static_cast<std::size_t>(kSuper) * kSuper;
@@ -807,6 +816,7 @@ int DenseCholesky<T>::factor(T* a, int m, const double* diag0, hipStream_t s) {
   // The side stream starts after everything already queued on s.
   SOV_HIP_TRY(hipEventRecord(ev_bulk_, s));
   SOV_HIP_TRY(hipStreamWaitEvent(side_, ev_bulk_, 0));
+  hipEvent_t bulk_done = ev_bulk_;  // completion of the last bulk update queued on the side stream
   const long long lda = m;
   const std::size_t pstride = static_cast<std::size_t>(m) * kPanel;
   for (int k0 = 0, p = 0; k0 < m; k0 += kPanel, ++p) {
@@ -834,29 +844,40 @@ int DenseCholesky<T>::factor(T* a, int m, const double* diag0, hipStream_t s) {
     // the strictly upper part of its diagonal block, which nothing reads.
     const int strip = std::min(kPanel, below);
     const int rest = below - strip;
-    SOV_HIP_TRY(hipStreamWaitEvent(s, ev_bulk_, 0));  // the strip was part of the previous bulk update
+    SOV_HIP_TRY(hipStreamWaitEvent(s, bulk_done, 0));  // the strip was part of the previous bulk update
     TimedPair* ts = timed_begin(s, kTimeStrip);
     T* a_strip = a + (k0 + nb) + static_cast<long long>(k0 + nb) * lda;
+    // ev_panel_ (the side stream's start, below) is recorded by the strip
+    // launch itself: a separate marker here would sit between the strip and
+    // the next panel kernel on the critical path.
     SOV_HIP_TRY((gemm_update_shape<T, false, false>(panel_shape, below, strip, nb, L21, lda, L21, lda, a_strip, lda,
-                                                    T(-1), s)));
+                                                    T(-1), s, ts ? nullptr : ev_panel_)));
     timed_end(ts, s, 1, 0.0);
+    if (ts) SOV_HIP_TRY(hipEventRecord(ev_panel_, s));
     // Side stream: the bulk update of the rest, then L21 back into the
-    // factor (only the triangular solves read it there). ev_bulk_ marks the
+    // factor (only the triangular solves read it there). bulk_done marks the
     // bulk update: the next strip waits for it. The L21 buffer of this panel
     // is written again kPanelBuffers = 3 panels later, by which time the main
     // stream has waited for the next panel's bulk update, queued after this
     // copy on the side stream.
-    SOV_HIP_TRY(hipEventRecord(ev_panel_, s));
     SOV_HIP_TRY(hipStreamWaitEvent(side_, ev_panel_, 0));
+    // The bulk launch records its completion itself (bulk_done: ev_bulk_, or
+    // the stop event of its timed pair), so no marker follows it.
     if (rest > 0) {
       T* a22 = a + (k0 + nb + strip) + static_cast<long long>(k0 + nb + strip) * lda;
       const T* l21r = L21 + strip;
-      TimedPair* tb = timed_begin(side_, sizeof(T) == 8 ? kTimeSyrk : kTimeSyrk32);
+      TimedPair* tb = timed_kernel(sizeof(T) == 8 ? kTimeSyrk : kTimeSyrk32);
+      bulk_done = tb ? tb->b : ev_bulk_;
       SOV_HIP_TRY((gemm_update_shape<T, false, true>(syrk_shape, rest, rest, nb, l21r, lda, l21r, lda, a22, lda, T(-1),
-                                                     side_)));
-      timed_end(tb, side_, 1, static_cast<double>(rest) * (rest + 1) * nb);
+                                                     side_, bulk_done, tb ? tb->a : nullptr)));
+      if (tb) {
+        tb->launches = 1;
+        tb->flops = static_cast<double>(rest) * (rest + 1) * nb;
+      }
+    } else {
+      SOV_HIP_TRY(hipEventRecord(ev_bulk_, side_));
+      bulk_done = ev_bulk_;
     }
-    SOV_HIP_TRY(hipEventRecord(ev_bulk_, side_));
     copy_panel_kernel<T><<<blocks_for(static_cast<long long>(below) * nb, 256), 256, 0, side_>>>(L21, a, lda, k0 + nb,
                                                                                                   k0, below, nb);
     SOV_LAUNCH_CHECK();
@@ -925,25 +946,28 @@ int DenseCholesky<T>::solve(const T* a, int m, double* x, double* /*work*/, hipS
   const long long lda = m, sq = static_cast<long long>(kSuper) * kSuper;
   const int nsb = (m + kSuper - 1) / kSuper;
   TimedPair* tp = timed_begin(s, kTimeSolve);
-  for (int P = 0; P < nsb; ++P) {  // L y = x: y into work_, x updated in place
-    const int base = P * kSuper, sz = std::min(kSuper, m - base), below = m - base - sz;
-    gemv_n_kernel<T, false, true><<<blocks_for(sz, kGemvRows), 64 * kGemvWaves, 0, s>>>(sz, sz, sup_inv_ + P * sq, kSuper,
-                                                                             x + base, work_ + base);
-    if (below > 0)
-      gemv_n_kernel<T, true, false><<<blocks_for(below, kGemvRows), 64 * kGemvWaves, 0, s>>>(
-          below, sz, a + (base + sz) + base * lda, lda, work_ + base, x + base + sz);
-  }
-  SOV_LAUNCH_CHECK();
-  for (int P = nsb - 1; P >= 0; --P) {  // L^T x = y: x back into x, work_ updated in place
-    const int base = P * kSuper, sz = std::min(kSuper, m - base);
-    gemv_t_kernel<T, false, true><<<blocks_for(sz, 4), 256, 0, s>>>(sz, sz, sup_inv_ + P * sq, kSuper, work_ + base,
-                                                                      x + base);
-    if (base > 0)
-      gemv_t_kernel<T, true, false><<<blocks_for(base, 4), 256, 0, s>>>(sz, base, a + base, lda, x + base, work_);
-  }
-  SOV_LAUNCH_CHECK();
+  const int rc = solve_graphs_.run(a, x, m, s, [&]() -> int {
+    for (int P = 0; P < nsb; ++P) {  // L y = x: y into work_, x updated in place
+      const int base = P * kSuper, sz = std::min(kSuper, m - base), below = m - base - sz;
+      gemv_n_kernel<T, false, true><<<blocks_for(sz, kGemvRows), 64 * kGemvWaves, 0, s>>>(
+          sz, sz, sup_inv_ + P * sq, kSuper, x + base, work_ + base);
+      if (below > 0)
+        gemv_n_kernel<T, true, false><<<blocks_for(below, kGemvRows), 64 * kGemvWaves, 0, s>>>(
+            below, sz, a + (base + sz) + base * lda, lda, work_ + base, x + base + sz);
+    }
+    SOV_LAUNCH_CHECK();
+    for (int P = nsb - 1; P >= 0; --P) {  // L^T x = y: x back into x, work_ updated in place
+      const int base = P * kSuper, sz = std::min(kSuper, m - base);
+      gemv_t_kernel<T, false, true><<<blocks_for(sz, 4), 256, 0, s>>>(sz, sz, sup_inv_ + P * sq, kSuper,
+                                                                        work_ + base, x + base);
+      if (base > 0)
+        gemv_t_kernel<T, true, false><<<blocks_for(base, 4), 256, 0, s>>>(sz, base, a + base, lda, x + base, work_);
+    }
+    SOV_LAUNCH_CHECK();
+    return SOV_OK;
+  });
   timed_end(tp, s, 4 * nsb, 0.0);
-  return SOV_OK;
+  return rc;
 }
 
 template class DenseCholesky<double>;

@@ -2,6 +2,7 @@
 #pragma once
 
 #include "common.hpp"
+#include "graphs.hpp"
 
 namespace sov {
 
@@ -38,6 +39,7 @@ class DenseCholesky {
   // elimination (for the drop test). Asynchronous; status in flags().
   int factor(T* a, int m, const double* diag0, hipStream_t s);
   // x <- L^-T L^-1 x on device vector x (FP64), using the factor in a.
+  // The launch sequence is replayed as a graph per (a, x, m).
   int solve(const T* a, int m, double* x, double* work, hipStream_t s);
   FactorFlags* flags() const { return flags_; }
 
@@ -52,6 +54,7 @@ class DenseCholesky {
   T* tmp_ = nullptr;      // products of the recursive doubling that builds them
   double* work_ = nullptr;  // intermediate vector of the solves (m)
   FactorFlags* flags_ = nullptr;
+  LaunchGraphs solve_graphs_;  // solve sequences replayed per (factor, vector, m)
 };
 
 // Global switch for per-kernel event timing (adds synchronisation) and the

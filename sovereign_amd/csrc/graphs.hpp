@@ -1,0 +1,94 @@
+// Replay of fixed launch sequences as HIP graphs.
+//
+// The solves of the interior-point step launch the same dozens of small,
+// dependent kernels over and over (a triangular solve pair is 32 GEMV
+// launches; a preconditioned-CG iteration adds ten more), and issued one by
+// one from the host they leave the GPU idle between launches. A sequence is
+// captured from its stream once per key (the buffers it reads and writes:
+// the graph holds their addresses) and later replayed with one graph launch.
+// The sequences hold kernels and device-to-device copies only (no memset
+// nodes, no host synchronisation). SOVEREIGN_GRAPHS=0 turns replay off; it is
+// also off while per-kernel event timing is on (timing level 1), so that the
+// event pairs see the individual launches.
+#pragma once
+
+#include "common.hpp"
+
+#include <cstdlib>
+#include <string>
+#include <vector>
+
+namespace sov {
+
+bool kernel_timing();
+
+inline bool graphs_enabled() {
+  const char* e = std::getenv("SOVEREIGN_GRAPHS");
+  return !(e && e[0] == '0') && !kernel_timing();
+}
+
+// True while s is being captured (work enqueued now becomes part of a graph).
+inline bool stream_capturing(hipStream_t s) {
+  hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
+  return hipStreamIsCapturing(s, &st) == hipSuccess && st == hipStreamCaptureStatusActive;
+}
+
+class LaunchGraphs {
+ public:
+  LaunchGraphs() = default;
+  LaunchGraphs(const LaunchGraphs&) = delete;
+  LaunchGraphs& operator=(const LaunchGraphs&) = delete;
+  ~LaunchGraphs() { clear(); }
+
+  void clear() {
+    for (Entry& e : entries_) (void)hipGraphExecDestroy(e.exec);
+    entries_.clear();
+  }
+
+  // Runs what enqueue() launches on s: eagerly when replay is off or s is
+  // already being captured (the launches then join the outer graph),
+  // otherwise as the graph captured for (k0, k1, k2), capturing it on first
+  // use. enqueue returns a status (SOV_OK on success).
+  template <typename F>
+  int run(const void* k0, const void* k1, long long k2, hipStream_t s, F&& enqueue) {
+    if (!graphs_enabled() || stream_capturing(s)) return enqueue();
+    for (const Entry& e : entries_)
+      if (e.k0 == k0 && e.k1 == k1 && e.k2 == k2) return hipGraphLaunch(e.exec, s) == hipSuccess ? SOV_OK : fail();
+    if (entries_.size() >= kMaxEntries) clear();
+    SOV_HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    const int rc = enqueue();
+    hipGraph_t g = nullptr;
+    const hipError_t ec = hipStreamEndCapture(s, &g);
+    if (rc != SOV_OK || ec != hipSuccess || !g) {
+      if (g) (void)hipGraphDestroy(g);
+      if (rc != SOV_OK) return rc;
+      set_error(std::string("graph capture failed: ") + hipGetErrorString(ec));
+      return SOV_ERR_DEVICE;
+    }
+    hipGraphExec_t exec = nullptr;
+    const hipError_t ei = hipGraphInstantiateWithFlags(&exec, g, 0);
+    (void)hipGraphDestroy(g);
+    if (ei != hipSuccess || !exec) {
+      set_error(std::string("graph instantiation failed: ") + hipGetErrorString(ei));
+      return SOV_ERR_DEVICE;
+    }
+    entries_.push_back(Entry{k0, k1, k2, exec});
+    return hipGraphLaunch(exec, s) == hipSuccess ? SOV_OK : fail();
+  }
+
+ private:
+  static constexpr std::size_t kMaxEntries = 8;
+  struct Entry {
+    const void* k0;
+    const void* k1;
+    long long k2;
+    hipGraphExec_t exec;
+  };
+  static int fail() {
+    set_error("graph launch failed");
+    return SOV_ERR_DEVICE;
+  }
+  std::vector<Entry> entries_;
+};
+
+}  // namespace sov

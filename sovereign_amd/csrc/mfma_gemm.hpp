@@ -21,6 +21,8 @@
 
 #include "common.hpp"
 
+#include <hip/hip_ext.h>
+
 namespace sov {
 
 typedef double dbl4 __attribute__((ext_vector_type(4)));
@@ -230,16 +232,27 @@ __global__ __launch_bounds__(256) void gemm_update_kernel(int M, int N, int K, c
 }
 
 // Host launcher: one product, or ny x nz of them offset by `bt`, with BM x BN
-// tiles. Returns a hipError_t from the launch.
+// tiles. `done` (and `start`), when given, are recorded by the launch itself
+// as the kernel's completion (and start): hipExtLaunchKernel's stop and
+// start events, which spare the stream separate markers around the kernel.
+// Returns a hipError_t from the launch.
 template <typename T, bool NN, bool LOWER, bool STORE = false, int BM = kTile, int BN = kTile>
 inline hipError_t gemm_update(int M, int N, int K, const T* A, long long lda, const T* B, long long ldb, T* C,
                               long long ldc, T alpha, hipStream_t s, int ny = 1, int nz = 1,
-                              GemmBatch bt = GemmBatch{}) {
-  if (M <= 0 || N <= 0 || K <= 0 || ny <= 0 || nz <= 0) return hipSuccess;
+                              GemmBatch bt = GemmBatch{}, hipEvent_t done = nullptr, hipEvent_t start = nullptr) {
+  if (M <= 0 || N <= 0 || K <= 0 || ny <= 0 || nz <= 0) {
+    if (start) (void)hipEventRecord(start, s);
+    return done ? hipEventRecord(done, s) : hipSuccess;
+  }
   const long long tm = (M + BM - 1) / BM, tn = (N + BN - 1) / BN;
   const long long tiles = LOWER ? tm * (tm + 1) / 2 : tm * tn;
   const dim3 grid(static_cast<unsigned>(tiles), static_cast<unsigned>(ny), static_cast<unsigned>(nz));
-  gemm_update_kernel<T, NN, LOWER, STORE, BM, BN><<<grid, 256, 0, s>>>(M, N, K, A, lda, B, ldb, C, ldc, alpha, bt);
+  if (done || start) {
+    hipExtLaunchKernelGGL(gemm_update_kernel<T, NN, LOWER, STORE, BM, BN>, grid, dim3(256), 0, s, start, done, 0, M,
+                          N, K, A, lda, B, ldb, C, ldc, alpha, bt);
+  } else {
+    gemm_update_kernel<T, NN, LOWER, STORE, BM, BN><<<grid, 256, 0, s>>>(M, N, K, A, lda, B, ldb, C, ldc, alpha, bt);
+  }
   return hipGetLastError();
 }
 
@@ -249,15 +262,20 @@ enum GemmShape { kGemm64x64 = 0, kGemm128x64 = 1, kGemm128x128 = 2, kGemmShapes 
 // gemm_update with a run-time tile shape (square shapes only when LOWER).
 template <typename T, bool NN, bool LOWER, bool STORE = false>
 inline hipError_t gemm_update_shape(int shape, int M, int N, int K, const T* A, long long lda, const T* B,
-                                    long long ldb, T* C, long long ldc, T alpha, hipStream_t s) {
+                                    long long ldb, T* C, long long ldc, T alpha, hipStream_t s,
+                                    hipEvent_t done = nullptr, hipEvent_t start = nullptr) {
+  const GemmBatch one{};
   switch (shape) {
     case kGemm128x128:
-      return gemm_update<T, NN, LOWER, STORE, 128, 128>(M, N, K, A, lda, B, ldb, C, ldc, alpha, s);
+      return gemm_update<T, NN, LOWER, STORE, 128, 128>(M, N, K, A, lda, B, ldb, C, ldc, alpha, s, 1, 1, one, done,
+                                                        start);
     case kGemm128x64:
-      if constexpr (!LOWER) return gemm_update<T, NN, LOWER, STORE, 128, 64>(M, N, K, A, lda, B, ldb, C, ldc, alpha, s);
-      else return gemm_update<T, NN, LOWER, STORE>(M, N, K, A, lda, B, ldb, C, ldc, alpha, s);
+      if constexpr (!LOWER)
+        return gemm_update<T, NN, LOWER, STORE, 128, 64>(M, N, K, A, lda, B, ldb, C, ldc, alpha, s, 1, 1, one, done,
+                                                         start);
+      else return gemm_update<T, NN, LOWER, STORE>(M, N, K, A, lda, B, ldb, C, ldc, alpha, s, 1, 1, one, done, start);
     default:
-      return gemm_update<T, NN, LOWER, STORE>(M, N, K, A, lda, B, ldb, C, ldc, alpha, s);
+      return gemm_update<T, NN, LOWER, STORE>(M, N, K, A, lda, B, ldb, C, ldc, alpha, s, 1, 1, one, done, start);
   }
 }
 

@@ -230,6 +230,7 @@ NormalSolver::~NormalSolver() {
 }
 
 int NormalSolver::init(sov_mat* a) {
+  pcg_graphs_.clear();
   A_ = a;
   m_ = a->nrows;
   n_ = a->ncols;
@@ -415,24 +416,39 @@ int NormalSolver::solve(double* x, hipStream_t s, double tol, int max_iter) {
   dot_kernel<<<rb, kRed, 0, s>>>(m_, r_, z_, partial_);
   dot_finish_kernel<<<1, kRed, 0, s>>>(rb, partial_, sc_, 0);
   SOV_LAUNCH_CHECK();
-  for (int it = 1; it <= max_iter; ++it) {
-    if ((rc = matvec(p_, q_, s)) != SOV_OK) return rc;
+  // The step of an iteration: q = M p, alpha, x += alpha p, r -= alpha q,
+  // and the residual norm the host reads.
+  auto step = [&]() -> int {
+    int r2 = matvec(p_, q_, s);
+    if (r2 != SOV_OK) return r2;
     dot_kernel<<<rb, kRed, 0, s>>>(m_, p_, q_, partial_);
     dot_finish_kernel<<<1, kRed, 0, s>>>(rb, partial_, sc_, 1);
     pcg_step_kernel<<<rb, kRed, 0, s>>>(m_, sc_, p_, q_, x, r_, partial_);
     max_finish_kernel<<<1, kRed, 0, s>>>(rb, partial_, sc_);
     SOV_LAUNCH_CHECK();
+    return SOV_OK;
+  };
+  if ((rc = step()) != SOV_OK) return rc;
+  for (int it = 1; it <= max_iter; ++it) {
     if ((rc = check(&ratio)) != SOV_OK) return rc;
     last_iters_ = it;
     last_res_ = ratio;
     if (!std::isfinite(ratio)) break;
     if (ratio <= tol) return SOV_OK;
-    SOV_HIP_TRY(hipMemcpyAsync(z_, r_, bytes, hipMemcpyDeviceToDevice, s));
-    if ((rc = ch32.solve(M32_, m_, z_, nullptr, s)) != SOV_OK) return rc;
-    dot_kernel<<<rb, kRed, 0, s>>>(m_, r_, z_, partial_);
-    dot_finish_kernel<<<1, kRed, 0, s>>>(rb, partial_, sc_, 2);
-    direction_kernel<<<rb, kRed, 0, s>>>(m_, sc_, 0, z_, p_);
-    SOV_LAUNCH_CHECK();
+    if (it == max_iter) break;
+    // Between two checks: z = P r, beta, p = z + beta p, then the next
+    // step; one graph launch per iteration (the solve is captured inline).
+    rc = pcg_graphs_.run(x, M32_, m_, s, [&]() -> int {
+      SOV_HIP_TRY(hipMemcpyAsync(z_, r_, bytes, hipMemcpyDeviceToDevice, s));
+      int r2 = ch32.solve(M32_, m_, z_, nullptr, s);
+      if (r2 != SOV_OK) return r2;
+      dot_kernel<<<rb, kRed, 0, s>>>(m_, r_, z_, partial_);
+      dot_finish_kernel<<<1, kRed, 0, s>>>(rb, partial_, sc_, 2);
+      direction_kernel<<<rb, kRed, 0, s>>>(m_, sc_, 0, z_, p_);
+      SOV_LAUNCH_CHECK();
+      return step();
+    });
+    if (rc != SOV_OK) return rc;
   }
   return SOV_SINGULAR;
 }

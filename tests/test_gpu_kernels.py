@@ -185,3 +185,35 @@ def test_normal_equations_solve(gpu_api, case, precision):
         assert np.max(np.abs(Mx - rhs)) <= 1e-8 * max(1.0, np.max(np.abs(rhs)))
         ne.close()
         A.close()
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_solve_graph_replays_match_eager(gpu_api, monkeypatch, precision):
+    """The solve sequences replayed as HIP graphs (the triangular solve pair;
+    for the FP32 factor, one CG iteration with its solve inside) give
+    bit-for-bit the eager launches' result on every replay, with eager work
+    (SpMV products) and host synchronisations between the replays, and again
+    after a refactorization with another diagonal. m = 2100 leaves a partial
+    last super-block."""
+    lp = P.random_sparse_lp(2100, 8400, 6, seed=9)
+    rng = np.random.default_rng(21)
+    A = gpu_api.SparseMatrix(lp.m, lp.n, lp.col_ptr, lp.row_idx, lp.values)
+    ne = gpu_api.NormalEquations(A)
+    for refactor in range(2):
+        d = np.exp(rng.uniform(-3, 3, size=lp.n))
+        assert ne.factor(d, precision)
+        for rep in range(4):
+            say(f"graph replay {precision} factorization {refactor} replay {rep}")
+            rhs = O.spmv(lp.m, lp.n, lp.col_ptr, lp.row_idx, lp.values, rng.standard_normal(lp.n))
+            monkeypatch.setenv("SOVEREIGN_GRAPHS", "1")
+            x_graph = ne.solve(rhs)
+            A.multiply(rng.standard_normal(lp.n))  # eager device work between the replays
+            monkeypatch.setenv("SOVEREIGN_GRAPHS", "0")
+            x_eager = ne.solve(rhs)
+            A.multiply_transpose(rng.standard_normal(lp.m))
+            np.testing.assert_array_equal(x_graph, x_eager)
+            at = O.spmv(lp.m, lp.n, lp.col_ptr, lp.row_idx, lp.values, x_graph, True)
+            Mx = O.spmv(lp.m, lp.n, lp.col_ptr, lp.row_idx, lp.values, d * at) + 1e-12 * x_graph
+            assert np.max(np.abs(Mx - rhs)) <= 1e-8 * max(1.0, np.max(np.abs(rhs)))
+    ne.close()
+    A.close()
