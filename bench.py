@@ -55,7 +55,7 @@ import numpy as np  # noqa: E402
 
 METRIC = "IPM solve s & strong-branch LP solves/s at 1e-8 gap vs plain FP64 GPU path"
 MFMA_PEAK = {"f64": 78.6, "f32": 157.3}  # TFLOP/s, MI355X dense matrix peaks
-SHAPES = {0: (64, 64), 1: (128, 64), 2: (128, 128)}
+SHAPES = {0: (64, 64), 1: (128, 64), 2: (128, 128), 3: (32, 32)}
 
 
 def kernel_name(dtype, shape):
@@ -67,15 +67,16 @@ def kernel_name(dtype, shape):
 def tuned_shapes(lib):
     """Tile shapes the factorization's autotuner picked, and its timings."""
     sh = np.full(4, -1, dtype=np.int32)
-    tm = np.zeros(10)
+    tm = np.zeros(12)
     lib.sov_gemm_shapes(sh.ctypes.data_as(C.POINTER(C.c_int32)), tm.ctypes.data_as(C.POINTER(C.c_double)))
-    names = ["64x64", "128x64", "128x128"]
+    names = ["64x64", "128x64", "128x128", "32x32"]
     out = {}
-    for p, (dt, off) in enumerate((("f64", 0), ("f32", 5))):
+    for p, (dt, off) in enumerate((("f64", 0), ("f32", 6))):
         out[dt] = {"syrk": names[sh[2 * p]] if sh[2 * p] >= 0 else None,
                    "panel": names[sh[2 * p + 1]] if sh[2 * p + 1] >= 0 else None,
                    "syrk_ms": {"64x64": tm[off], "128x128": tm[off + 1]},
-                   "panel_ms": {"64x64": tm[off + 2], "128x64": tm[off + 3], "128x128": tm[off + 4]}}
+                   "panel_ms": {"64x64": tm[off + 2], "128x64": tm[off + 3], "128x128": tm[off + 4],
+                                "32x32": tm[off + 5]}}
     return out, sh
 NT = 12  # slots of sov_kernel_times
 

@@ -81,13 +81,13 @@ void sov_set_kernel_timing(int32_t on);
 int sov_kernel_times(double* out12);
 
 /* MFMA tile shapes chosen by the factorization's autotuner (0: 64x64,
- * 1: 128x64, 2: 128x128; -1 before the first factorization in that
- * precision): shapes4 = {FP64 SYRK, FP64 panel products, FP32 SYRK, FP32
- * panel products}. times_ms10 (may be NULL) receives the tuning runs' times
- * per launch: FP64 SYRK 64x64, 128x128, FP64 panel 64x64, 128x64, 128x128,
- * then the same five for FP32. SOVEREIGN_GEMM_SHAPE=0/1/2 overrides the
- * choice. No counterpart in the reference. */
-int sov_gemm_shapes(int32_t* shapes4, double* times_ms10);
+ * 1: 128x64, 2: 128x128, 3: 32x32; -1 before the first factorization in
+ * that precision): shapes4 = {FP64 SYRK, FP64 panel products, FP32 SYRK,
+ * FP32 panel products}. times_ms12 (may be NULL) receives the tuning runs'
+ * times per launch: FP64 SYRK 64x64, 128x128, FP64 panel 64x64, 128x64,
+ * 128x128, 32x32, then the same six for FP32. SOVEREIGN_GEMM_SHAPE=0/1/2/3
+ * overrides the choice. No counterpart in the reference. */
+int sov_gemm_shapes(int32_t* shapes4, double* times_ms12);
 
 /* -------------------------------------------------------------- dense LU --
  * Replaces cuda::dense_lu (cuda_driver.hpp:29-34; run_dense_lu at

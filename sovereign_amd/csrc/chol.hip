@@ -721,15 +721,17 @@ int DenseCholesky<T>::reserve(int m) {
 // Tile-shape autotuner for the two MFMA products of the factorization: the
 // bulk trailing SYRK (lower, square tiles: 64x64 or 128x128) and the panel
 // products (TRSM as a GEMM and the strip update, M x 128 x 128: 64x64,
-// 128x64 or 128x128). On the first factorization in a precision each shape
-// runs a product of the class at a representative size (SYRK 4096 x 4096 x
-// 128, panel 8192 x 128 x 128), one warm-up and five timed launches (HIP
-// events, the fastest counts), and the fastest shape is kept for the process. SOVEREIGN_GEMM_SHAPE =
-// 0 / 1 / 2 forces one shape (128x64 falls back to 64x64 for the SYRK).
+// 128x64, 128x128 or 32x32). On the first factorization in a precision each
+// shape runs a product of the class at a representative size (SYRK 4096 x
+// 4096 x 128; panel 4096 x 128 x 128, the middle of the factorization's
+// 8064 .. 128 rows), one warm-up and five timed launches (HIP events, the
+// fastest counts), and the fastest shape is kept for the process.
+// SOVEREIGN_GEMM_SHAPE = 0 / 1 / 2 / 3 forces one shape (128x64 falls back
+// to 64x64 for the SYRK).
 // ---------------------------------------------------------------------------
 struct TunedShapes {
   std::atomic<int> syrk{-1}, panel{-1};
-  double syrk_ms[kGemmShapes] = {0, 0, 0}, panel_ms[kGemmShapes] = {0, 0, 0};
+  double syrk_ms[kGemmShapes] = {0, 0, 0, 0}, panel_ms[kGemmShapes] = {0, 0, 0, 0};
 };
 template <typename T>
 TunedShapes& tuned() {
@@ -783,9 +785,9 @@ int autotune_shapes(hipStream_t s) {
     });
     if (t.syrk_ms[shape] < t.syrk_ms[best_s]) best_s = shape;
   }
-  for (int shape : {kGemm64x64, kGemm128x64, kGemm128x128}) {
+  for (int shape : {kGemm64x64, kGemm128x64, kGemm128x128, kGemm32x32}) {
     t.panel_ms[shape] = time_it([&] {
-      (void)gemm_update_shape<T, false, false>(shape, mp, k, k, a, mp, a, mp, c, mp, T(-1), s);
+      (void)gemm_update_shape<T, false, false>(shape, mp / 2, k, k, a, mp, a, mp, c, mp, T(-1), s);
     });
     if (t.panel_ms[shape] < t.panel_ms[best_p]) best_p = shape;
   }
@@ -977,18 +979,18 @@ template class DenseCholesky<float>;
 
 extern "C" {
 
-int sov_gemm_shapes(int32_t* shapes4, double* times_ms10) {
+int sov_gemm_shapes(int32_t* shapes4, double* times_ms12) {
   if (!shapes4) return SOV_ERR_ARG;
   shapes4[0] = sov::tuned<double>().syrk.load();
   shapes4[1] = sov::tuned<double>().panel.load();
   shapes4[2] = sov::tuned<float>().syrk.load();
   shapes4[3] = sov::tuned<float>().panel.load();
-  if (times_ms10) {  // f64 SYRK 64x64, 128x128; f64 panel 64x64, 128x64, 128x128; the same for f32
+  if (times_ms12) {  // f64 SYRK 64x64, 128x128; f64 panel 64x64, 128x64, 128x128, 32x32; the same for f32
     const sov::TunedShapes* t[2] = {&sov::tuned<double>(), &sov::tuned<float>()};
     for (int p = 0; p < 2; ++p) {
-      times_ms10[5 * p + 0] = t[p]->syrk_ms[sov::kGemm64x64];
-      times_ms10[5 * p + 1] = t[p]->syrk_ms[sov::kGemm128x128];
-      for (int k = 0; k < 3; ++k) times_ms10[5 * p + 2 + k] = t[p]->panel_ms[k];
+      times_ms12[6 * p + 0] = t[p]->syrk_ms[sov::kGemm64x64];
+      times_ms12[6 * p + 1] = t[p]->syrk_ms[sov::kGemm128x128];
+      for (int k = 0; k < 4; ++k) times_ms12[6 * p + 2 + k] = t[p]->panel_ms[k];
     }
   }
   return SOV_OK;

@@ -9,8 +9,8 @@
 // rank-K update of a lower triangle) and launches only tiles on or below the
 // diagonal.
 //
-// Tiling for CDNA4: a 256-thread workgroup owns a BM x BN tile of C (64x64,
-// 128x64 or 128x128) as a 2x2 grid of wavefronts, each a grid of 16x16 MFMA
+// Tiling for CDNA4: a 256-thread workgroup owns a BM x BN tile of C (32x32,
+// 64x64, 128x64 or 128x128) as a 2x2 grid of wavefronts, each a grid of 16x16 MFMA
 // blocks (v_mfma_f64_16x16x4f64 / v_mfma_f32_16x16x4f32). K is staged through
 // LDS in chunks of 32 with a register prefetch of the next chunk. LDS rows are
 // padded by 16 elements so the four k-rows one MFMA operand read touches fall
@@ -260,7 +260,9 @@ inline hipError_t gemm_update(int M, int N, int K, const T* A, long long lda, co
 }
 
 // Tile shapes the autotuner chooses between for a given product class.
-enum GemmShape { kGemm64x64 = 0, kGemm128x64 = 1, kGemm128x128 = 2, kGemmShapes = 3 };
+// 32x32 suits the latency-bound panel products (their time is set by one
+// tile's MFMA chain and loads, not by their width).
+enum GemmShape { kGemm64x64 = 0, kGemm128x64 = 1, kGemm128x128 = 2, kGemm32x32 = 3, kGemmShapes = 4 };
 
 // gemm_update with a run-time tile shape (square shapes only when LOWER).
 template <typename T, bool NN, bool LOWER, bool STORE = false>
@@ -272,6 +274,9 @@ inline hipError_t gemm_update_shape(int shape, int M, int N, int K, const T* A, 
     case kGemm128x128:
       return gemm_update<T, NN, LOWER, STORE, 128, 128>(M, N, K, A, lda, B, ldb, C, ldc, alpha, s, 1, 1, one, done,
                                                         start);
+    case kGemm32x32:
+      return gemm_update<T, NN, LOWER, STORE, 32, 32>(M, N, K, A, lda, B, ldb, C, ldc, alpha, s, 1, 1, one, done,
+                                                      start);
     case kGemm128x64:
       if constexpr (!LOWER)
         return gemm_update<T, NN, LOWER, STORE, 128, 64>(M, N, K, A, lda, B, ldb, C, ldc, alpha, s, 1, 1, one, done,
