@@ -948,7 +948,7 @@ int DenseCholesky<T>::solve(const T* a, int m, double* x, double* /*work*/, hipS
   const long long lda = m, sq = static_cast<long long>(kSuper) * kSuper;
   const int nsb = (m + kSuper - 1) / kSuper;
   TimedPair* tp = timed_begin(s, kTimeSolve);
-  const int rc = solve_graphs_.run(a, x, m, s, [&]() -> int {
+  const int rc = solve_graphs_.run(a, x, nullptr, m, s, [&]() -> int {
     for (int P = 0; P < nsb; ++P) {  // L y = x: y into work_, x updated in place
       const int base = P * kSuper, sz = std::min(kSuper, m - base), below = m - base - sz;
       gemv_n_kernel<T, false, true><<<blocks_for(sz, kGemvRows), 64 * kGemvWaves, 0, s>>>(

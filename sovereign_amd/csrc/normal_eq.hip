@@ -438,7 +438,7 @@ int NormalSolver::solve(double* x, hipStream_t s, double tol, int max_iter) {
     if (it == max_iter) break;
     // Between two checks: z = P r, beta, p = z + beta p, then the next
     // step; one graph launch per iteration (the solve is captured inline).
-    rc = pcg_graphs_.run(x, M32_, m_, s, [&]() -> int {
+    rc = pcg_graphs_.run(x, M32_, d_, m_, s, [&]() -> int {  // d_: the product M p reads it
       SOV_HIP_TRY(hipMemcpyAsync(z_, r_, bytes, hipMemcpyDeviceToDevice, s));
       int r2 = ch32.solve(M32_, m_, z_, nullptr, s);
       if (r2 != SOV_OK) return r2;

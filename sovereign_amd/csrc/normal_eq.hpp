@@ -65,7 +65,7 @@ class NormalSolver {
   double* partial_ = nullptr;
   PcgScalars* sc_ = nullptr;
   PcgScalars* sc_host_ = nullptr;
-  LaunchGraphs pcg_graphs_;  // one CG iteration's launches, replayed per solution vector
+  LaunchGraphs pcg_graphs_;  // one CG iteration's launches, replayed per (solution vector, d)
   int prec_ = 0;
   int last_iters_ = 0;
   double last_res_ = 0.0;
