@@ -33,7 +33,12 @@ typedef double dbl4_t __attribute__((ext_vector_type(4)));
 
 namespace {
 
+// 16 wavefronts. At this size the kernel spills about 850 B per thread (128
+// registers a wave); 512 threads remove the spills but take longer per LP
+// (85 against 77 ms for the 128 knapsack children): the LDS factorization
+// and inversion steps have half the threads.
 constexpr int kThreads = 1024;
+static_assert(kThreads % 256 == 0 && kThreads <= 1024, "gemv_n slices the columns in 256-row groups");
 constexpr int kWaves = kThreads / 64;
 constexpr int kMaxRows = 272;
 constexpr int kStage = 1024;  // doubles of LDS staging (vectors, partial sums)
@@ -143,18 +148,21 @@ __device__ void gemv_t(const Lp& L, const double* vec, double* out, double* stag
 
 // out_i = base_i + (A vec)_i for i < m (base may be null). vec (n) global.
 __device__ void gemv_n(const Lp& L, const double* vec, const double* base, double* out, double* stage) {
+  constexpr int kSlices = kThreads / 256;  // slices of the columns, 256 rows each
   const int tid = threadIdx.x;
-  const int row = tid & 255, h = tid >> 8;  // 4 slices of the columns
+  const int row = tid & 255, h = tid >> 8;
   for (int r0 = 0; r0 < L.m; r0 += 256) {
     const int i = r0 + row;
     double acc = 0.0;
     if (i < L.m)
-      for (int j = h; j < L.n; j += 4) acc = fma(L.A[i + static_cast<long long>(j) * L.m], vec[j], acc);
+      for (int j = h; j < L.n; j += kSlices) acc = fma(L.A[i + static_cast<long long>(j) * L.m], vec[j], acc);
     __syncthreads();
     stage[h * 256 + row] = acc;
     __syncthreads();
     if (h == 0 && i < L.m) {
-      const double s = stage[row] + stage[256 + row] + stage[512 + row] + stage[768 + row];
+      double s = stage[row];
+#pragma unroll
+      for (int q = 1; q < kSlices; ++q) s += stage[q * 256 + row];
       out[i] = (base ? base[i] : 0.0) + s;
     }
   }
