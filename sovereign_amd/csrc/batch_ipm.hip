@@ -547,8 +547,7 @@ __global__ __launch_bounds__(kThreads) void batch_ipm_kernel(const BatchMatrix* 
     auto note = [&](int it, int col, double v) {
       if (tr && tid == 0 && it < kTraceRows) tr[it * kTraceCols + col] = v;
     };
-    int stalled = 0;  // consecutive iterations whose two PCG solves both hit the cap
-    int last_pcg = 0;
+    int stalled = 0;  // consecutive FP64 iterations whose two PCG solves both hit the cap
     bool fp64 = false;  // FP64 factorizations (global memory) after the FP32 solves stalled
     int fp64_from = -1;
     double best = 1e300;
@@ -612,10 +611,11 @@ __global__ __launch_bounds__(kThreads) void batch_ipm_kernel(const BatchMatrix* 
         status = 6;
         break;
       }
-      // When the FP32-preconditioned solves stall (both hit their cap in an
-      // iteration, or 30 iterations pass without progress) the LP switches to
-      // FP64 factorizations; stalling again in FP64 (three iterations) ends
-      // it with status 6, which the caller finishes on the per-LP path.
+      // When an FP32-preconditioned solve hits its cap (below, where the
+      // iteration is then redone) or 30 iterations pass without progress the
+      // LP switches to FP64 factorizations; stalling in FP64 (both solves at
+      // the cap in three iterations) ends it with status 6, which the caller
+      // finishes on the per-LP path.
       const double metric = fmax(pres, fmax(dres, gap));
       if (metric < best) {
         best = metric;
@@ -649,64 +649,82 @@ __global__ __launch_bounds__(kThreads) void batch_ipm_kernel(const BatchMatrix* 
       __syncthreads();
       long long t1 = clock64();
       note(it, 9, static_cast<double>(t1 - t0));
-      if (fp64) assemble(L, L.M64, 0.0);
-      else assemble(L, M, 0.0);
-      t0 = clock64();
-      note(it, 10, static_cast<double>(t0 - t1));
-      note(it, 15, fp64 ? cholesky(L, L.M64) : cholesky(L, M));
-      t1 = clock64();
-      note(it, 11, static_cast<double>(t1 - t0));
-      if (fp64) invert_lower(L, L.M64, stage);
-      else invert_lower(L, M, stage);
-      // inexact Newton while mu is large: CG relative residual max(pcg_tol, min(1e-6, 1e-3 mu))
-      const double ptol = fp64 ? o.pcg_tol : fmax(o.pcg_tol, fmin(1e-6, 1e-3 * mu));
-      t0 = clock64();
-      note(it, 12, static_cast<double>(t0 - t1));
-      // predictor: r1 = -x z, r2 = -w v (in dxa/dwa scratch: use dza/dva as rhs storage)
-      for (int j = tid; j < n; j += kThreads) {
-        L.dza[j] = -L.x[j] * L.z[j];
-        L.dva[j] = -L.w[j] * L.v[j];
-      }
-      __syncthreads();
-      // r1 -> p-buffer copies: newton reads r1/r2 after writing dz/dv, so pass copies.
-      for (int j = tid; j < n; j += kThreads) {
-        L.dxa[j] = L.dza[j];
-        L.dwa[j] = L.dva[j];
-      }
-      __syncthreads();
-      {
+      double ap, ad;
+      // Factor and solve; an FP32-preconditioned solve that hits the CG cap
+      // is not used: the LP switches to FP64 factorizations and the
+      // iteration is redone (a step along the unconverged direction can undo
+      // the primal feasibility of a nearly solved LP).
+      for (;;) {
+        if (fp64) assemble(L, L.M64, 0.0);
+        else assemble(L, M, 0.0);
+        t0 = clock64();
+        note(it, 10, static_cast<double>(t0 - t1));
+        note(it, 15, fp64 ? cholesky(L, L.M64) : cholesky(L, M));
+        t1 = clock64();
+        note(it, 11, static_cast<double>(t1 - t0));
+        if (fp64) invert_lower(L, L.M64, stage);
+        else invert_lower(L, M, stage);
+        // inexact Newton while mu is large: CG relative residual max(pcg_tol, min(1e-6, 1e-3 mu))
+        const double ptol = fp64 ? o.pcg_tol : fmax(o.pcg_tol, fmin(1e-6, 1e-3 * mu));
+        t0 = clock64();
+        note(it, 12, static_cast<double>(t0 - t1));
+        // predictor: r1 = -x z, r2 = -w v (in dxa/dwa scratch: use dza/dva as rhs storage)
+        for (int j = tid; j < n; j += kThreads) {
+          L.dza[j] = -L.x[j] * L.z[j];
+          L.dva[j] = -L.w[j] * L.v[j];
+        }
+        __syncthreads();
+        // r1 -> p-buffer copies: newton reads r1/r2 after writing dz/dv, so pass copies.
+        for (int j = tid; j < n; j += kThreads) {
+          L.dxa[j] = L.dza[j];
+          L.dwa[j] = L.dva[j];
+        }
+        __syncthreads();
         const int k1 = fp64 ? newton(L, L.M64, stage, red, L.dxa, L.dwa, L.dx, L.dw, L.dz, L.dv, ptol, o)
                             : newton(L, M, stage, red, L.dxa, L.dwa, L.dx, L.dw, L.dz, L.dv, ptol, o);
         pcg_total += k1;
-        last_pcg = k1;
         note(it, 7, k1);
         t1 = clock64();
         note(it, 13, static_cast<double>(t1 - t0));
-      }
-      double ap, ad;
-      max_steps(L, L.dx, L.dw, L.dz, L.dv, red, &ap, &ad);
-      double mua = 0.0;
-      for (int j = tid; j < n; j += kThreads) {
-        if (!active(L, j)) continue;
-        mua += (L.x[j] + ap * L.dx[j]) * (L.z[j] + ad * L.dz[j]);
-        if (bounded(L, j)) mua += (L.w[j] + ap * L.dw[j]) * (L.v[j] + ad * L.dv[j]);
-      }
-      mua = bred<0>(mua, red) / fmax(nact + nb, 1.0);
-      const double sigma = mu > 0.0 ? fmin(1.0, pow(mua / mu, 3.0)) : 0.0;
-      // corrector: r1 = sigma mu - x z - dx_a dz_a, r2 = sigma mu - w v - dw_a dv_a
-      for (int j = tid; j < n; j += kThreads) {
-        L.dxa[j] = sigma * mu - L.x[j] * L.z[j] - L.dx[j] * L.dz[j];
-        L.dwa[j] = bounded(L, j) ? sigma * mu - L.w[j] * L.v[j] - L.dw[j] * L.dv[j] : 0.0;
-      }
-      __syncthreads();
-      {
+        if (!fp64 && k1 >= o.pcg_max) {
+          fp64 = true;
+          fp64_from = it;
+          stalled = 0;
+          best_it = it;
+          note(it, 15, -1.0);
+          continue;
+        }
+        max_steps(L, L.dx, L.dw, L.dz, L.dv, red, &ap, &ad);
+        double mua = 0.0;
+        for (int j = tid; j < n; j += kThreads) {
+          if (!active(L, j)) continue;
+          mua += (L.x[j] + ap * L.dx[j]) * (L.z[j] + ad * L.dz[j]);
+          if (bounded(L, j)) mua += (L.w[j] + ap * L.dw[j]) * (L.v[j] + ad * L.dv[j]);
+        }
+        mua = bred<0>(mua, red) / fmax(nact + nb, 1.0);
+        const double sigma = mu > 0.0 ? fmin(1.0, pow(mua / mu, 3.0)) : 0.0;
+        // corrector: r1 = sigma mu - x z - dx_a dz_a, r2 = sigma mu - w v - dw_a dv_a
+        for (int j = tid; j < n; j += kThreads) {
+          L.dxa[j] = sigma * mu - L.x[j] * L.z[j] - L.dx[j] * L.dz[j];
+          L.dwa[j] = bounded(L, j) ? sigma * mu - L.w[j] * L.v[j] - L.dw[j] * L.dv[j] : 0.0;
+        }
+        __syncthreads();
         const int k2 = fp64 ? newton(L, L.M64, stage, red, L.dxa, L.dwa, L.dx, L.dw, L.dz, L.dv, ptol, o)
                             : newton(L, M, stage, red, L.dxa, L.dwa, L.dx, L.dw, L.dz, L.dv, ptol, o);
         pcg_total += k2;
         note(it, 8, k2);
-        stalled = (k2 >= o.pcg_max && last_pcg >= o.pcg_max) ? stalled + 1 : 0;
         t0 = clock64();
         note(it, 14, static_cast<double>(t0 - t1));
+        if (!fp64 && k2 >= o.pcg_max) {
+          fp64 = true;
+          fp64_from = it;
+          stalled = 0;
+          best_it = it;
+          note(it, 15, -1.0);
+          continue;
+        }
+        stalled = (k2 >= o.pcg_max && k1 >= o.pcg_max) ? stalled + 1 : 0;
+        break;
       }
       max_steps(L, L.dx, L.dw, L.dz, L.dv, red, &ap, &ad);
       ap = fmin(1.0, o.tau * ap);
