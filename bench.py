@@ -34,6 +34,7 @@ n=20000 instance), with the oracle's one-thread solve as its CPU baseline.
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
                        [--paths mixed,fp64] [--no-branching] [--no-qp]
        python bench.py --only-qp [--steps K] [--no-cpu]   (the QP leg alone)
+       python bench.py --only-branching   (the strong-branching leg alone)
        python bench.py --copy-probe   (a plain 256 MiB device copy, three
                                        times: the calibration run for the
                                        FETCH_SIZE / WRITE_SIZE counters)
@@ -293,6 +294,7 @@ def main():
     ap.add_argument("--no-branching", action="store_true")
     ap.add_argument("--no-qp", action="store_true")
     ap.add_argument("--only-qp", action="store_true", help="run the separable QP leg alone (profiling)")
+    ap.add_argument("--only-branching", action="store_true", help="run the strong-branching leg alone (profiling)")
     ap.add_argument("--paths", default="mixed,fp64")
     ap.add_argument("--m", type=int, default=8192)
     ap.add_argument("--n", type=int, default=32768)
@@ -310,8 +312,7 @@ def main():
         print(json.dumps({"copy_probe_bytes": x.numel() * 4, "copies": 3}))
         return
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path (the sharding design is in DESIGN.md).",
-              file=sys.stderr)
+        print("bench.py measures one GPU: the multi-GPU (sharded) path is designed but not built.", file=sys.stderr)
         sys.exit(2)
     paths = [p for p in args.paths.split(",") if p]
     if args.only_qp:
@@ -319,6 +320,11 @@ def main():
 
         _capi.load()
         print(json.dumps({"separable_qp": qp_leg(max(1, args.steps), not args.no_cpu)}), flush=True)
+        return
+    if args.only_branching:
+        from sovereign_amd import _capi
+
+        print(json.dumps({"strong_branching": branching_leg(_capi.load())}), flush=True)
         return
 
     from sovereign_amd import _capi, api, problems as P
