@@ -365,7 +365,8 @@ def main():
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": wall / args.steps * 1e3,
         "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
-        "dtype": ("f32 MFMA factorizations while mu >= 1e-6, f64 residuals/PCG/late factorizations"
+        "dtype": ("f32 MFMA factorizations until the f32-preconditioned PCG passes 8 iterations per IPM "
+                  "iteration (or mu < 1e-6), f64 residuals/PCG/late factorizations"
                   if main_path == "mixed" else "f64"),
         "data": "synthetic",
         "config": {"workload": f"random sparse LP m={lp.m} n={lp.n} 6 nnz/col seed {P.BASE_SEED}",

@@ -154,7 +154,9 @@ typedef struct {
   int32_t enable_scaling;      /* 1 (general-form solve only) */
   /* SOV_PREC_FP64: every factorization in FP64 (the plain FP64 GPU path).
    * SOV_PREC_FP32: mixed precision - FP32 factorizations while
-   * mu >= mixed_switch_mu, FP64 afterwards; FP64 residuals and refinement. */
+   * mu >= mixed_switch_mu and the FP32-preconditioned CG of an iteration
+   * takes at most 8 iterations (and has not stalled), FP64 afterwards;
+   * FP64 residuals and refinement. */
   int32_t precision;
   double mixed_switch_mu; /* 1e-6 */
   int32_t refine_max;     /* 30: PCG iterations per solve at most */

@@ -25,6 +25,10 @@ namespace sov {
 namespace {
 
 constexpr int kThreads = 256;
+// CG iterations (predictor + corrector) of one FP32-preconditioned IPM
+// iteration past which the remaining factorizations are FP64: the next
+// iteration's count, about twice this one, would pass the break-even below.
+constexpr int kPcgLockIterations = 8;
 
 struct IpmScalars {
   double pres, dres, gap, mu, xs, cx;
@@ -528,7 +532,11 @@ int ipm_run(sov_ipm* h, const double* b_h, const double* c_h, const sov_ipm_opti
     ++res->factorizations;
     if (cur_prec == SOV_PREC_FP32) {
       ++res->fp32_factorizations;
-      if (ne.breakdown(st) != SOV_OK) {  // FP32 breakdown: redo this one in FP64
+      const int broke = ne.breakdown(st);
+      if (trace)
+        std::fprintf(stderr, "[ipm] it=%d fp32 factor: replaced pivots %d%s\n", it, ne.replaced_pivots(),
+                     broke != SOV_OK ? ", breakdown (redone in fp64)" : "");
+      if (broke != SOV_OK) {  // FP32 breakdown: redo this one in FP64
         fp64_locked = true;
         cur_prec = SOV_PREC_FP64;
         if ((rc = ne.factor(d.p, cur_prec, st)) != SOV_OK) return rc;
@@ -541,6 +549,7 @@ int ipm_run(sov_ipm* h, const double* b_h, const double* c_h, const sov_ipm_opti
     }
 
     // Affine predictor.
+    const int pcg_before = res->refine_iterations;
     launch_rows_axpy(tmp.p, rp.p, dy.p);
     SOV_LAUNCH_CHECK();
     (void)hipEventRecord(ev.e[2], st);
@@ -571,6 +580,14 @@ int ipm_run(sov_ipm* h, const double* b_h, const double* c_h, const sov_ipm_opti
       finished = true;
       break;
     }
+    // The preconditioned CG's iteration count roughly doubles per
+    // iteration once mu is small. An FP32 factorization plus the CG costs
+    // more than an FP64 factorization with direct solves past about 11 CG
+    // iterations (m = 8192: 5.2 ms + 0.28 ms per CG iteration against
+    // 8.3 ms), so once the next iteration is expected past that, the
+    // remaining factorizations are FP64, before a CG stall forces one to be
+    // redone.
+    if (cur_prec == SOV_PREC_FP32 && res->refine_iterations - pcg_before > kPcgLockIterations) fp64_locked = true;
     pending_timing = true;
     launch_direction(dy.p, dx.p, ds.p);
     alpha_finish_kernel<<<1, kThreads, 0, st>>>(col_blocks, pcol.p, tau, 1, sc);
