@@ -138,8 +138,15 @@ struct LeafMath<float> {
 // D[i * kPanelLd + k]) and 1 / L(j, j) to xd[j]; counts dropped pivots and
 // breakdowns into flags once at the end. thr (lane j): drop threshold of
 // column j. Pivot rule as described above the kernel.
-template <typename T>
-__device__ __forceinline__ void leaf_factor(T* D, int h, T thr, T* xd, FactorFlags* flags, int lane) {
+//
+// EXACT = false is the fast pass: it takes every pivot as is and, if any
+// pivot would have been dropped or broken down, returns false having written
+// nothing (the caller then runs the EXACT pass, which applies the rule).
+// Where no pivot needs the rule both passes compute the same values; the
+// fast pass saves the rule's per-step selects (a step is issue-bound: ~96
+// wave-wide instructions in FP64).
+template <typename T, bool EXACT>
+__device__ __forceinline__ bool leaf_factor_pass(T* D, int h, T thr, T* xd, FactorFlags* flags, int lane) {
   constexpr int NR = kLeaf / 8;
   // The lane masks of the steps are formed from an opaque copy of the lane
   // index, so that they are computed here and not hoisted out of the
@@ -160,21 +167,27 @@ __device__ __forceinline__ void leaf_factor(T* D, int h, T thr, T* xd, FactorFla
   }
   T xdr = T(1);  // lane j: 1 / L(j, j)
   int ndrop = 0;
-  bool anybad = false;
+  bool anybad = false, plain = true;
   T dnext = read_lane(f[0][0], 0);
 #pragma unroll
   for (int j = 0; j < kLeaf; ++j) {
     if (j < h) {  // (the guard also keeps the steps from being interleaved, which costs registers)
       const int jp = j & 7, js = j >> 3;
       const T dj = dnext;
-      const bool finite = isfinite(dj);
-      const bool drop = finite & (dj <= read_lane(thr, j));
-      const bool bad = !drop & (!finite | (dj < T(1e-30)));
-      ndrop += drop ? 1 : 0;
-      anybad = anybad || bad;
       const T rs = LeafMath<T>::rsqrt(dj);
-      const T piv = drop ? DropPivot<T>::value : bad ? T(1) : dj * rs;
-      const T rpiv = drop ? T(1) / DropPivot<T>::value : bad ? T(1) : rs;
+      T piv = dj * rs, rpiv = rs;
+      if constexpr (EXACT) {
+        const bool finite = isfinite(dj);
+        const bool drop = finite & (dj <= read_lane(thr, j));
+        const bool bad = !drop & (!finite | (dj < T(1e-30)));
+        ndrop += drop ? 1 : 0;
+        anybad = anybad || bad;
+        piv = drop ? DropPivot<T>::value : bad ? T(1) : piv;
+        rpiv = drop ? T(1) / DropPivot<T>::value : bad ? T(1) : rpiv;
+      } else {
+        // the negation of (dropped or broken down) in the rule above
+        plain = plain & (dj > read_lane(thr, j)) & (dj >= T(1e-30)) & isfinite(dj);
+      }
       xdr = ln == j ? rpiv : xdr;
       if (ln == 9 * jp) dg[js] = piv;
       if (q == jp) {  // column j of L: a / pivot below the diagonal, zero on and above it
@@ -208,6 +221,9 @@ __device__ __forceinline__ void leaf_factor(T* D, int h, T thr, T* xd, FactorFla
         }
     }
   }
+  if constexpr (!EXACT) {
+    if (!plain) return false;  // (uniform: every lane tests the same pivots)
+  }
 #pragma unroll
   for (int r = 0; r < NR; ++r)
 #pragma unroll
@@ -217,10 +233,16 @@ __device__ __forceinline__ void leaf_factor(T* D, int h, T thr, T* xd, FactorFla
       if (i < h && k < i) D[i * kPanelLd + k] = x[r][s2];                    // X(i, k), above the diagonal
     }
   if (ln < h) xd[ln] = xdr;
-  if (ln == 0) {
+  if (EXACT && ln == 0) {
     if (ndrop) atomicAdd(&flags->replaced, ndrop);
     if (anybad) atomicOr(&flags->status, 1);
   }
+  return true;
+}
+
+template <typename T>
+__device__ __forceinline__ void leaf_factor(T* D, int h, T thr, T* xd, FactorFlags* flags, int lane) {
+  if (!leaf_factor_pass<T, false>(D, h, thr, xd, flags, lane)) leaf_factor_pass<T, true>(D, h, thr, xd, flags, lane);
 }
 
 // Phase A for sub-panel c (width sb) by one wavefront: the two 16 x 16

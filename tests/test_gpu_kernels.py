@@ -187,6 +187,35 @@ def test_normal_equations_solve(gpu_api, case, precision):
         A.close()
 
 
+def test_normal_equations_dropped_pivot(gpu_api):
+    """A balanced transport problem has one redundant row, so M = A D A^T
+    is singular up to the 1e-12 regularization. With D ~ 1e3 that row's
+    FP64 pivot falls under the drop threshold 1e-13 |d0|: the factorization
+    must replace it (the pivot rule of sparse_ldlt.cpp:165-173), count it,
+    and still solve consistent right-hand sides (checked on A^T dy)."""
+    model = P.transport_balanced(20, 20)
+    f = O.ipm_form(model)
+    m_, n_, cp, ri, va = f["m"], f["n"], f["col_ptr"], f["row_idx"], f["values"]
+    rng = np.random.default_rng(11)
+    d = 1e3 * np.exp(rng.uniform(-1, 1, size=n_))
+    rhs = O.spmv(m_, n_, cp, ri, va, rng.standard_normal(n_))
+    ne_c = O.NormalEquations(m_, n_, cp, ri, va, "sparse")  # the LDL^T with the pivot rule
+    assert ne_c.factor(d)
+    assert ne_c.replaced_pivots == 1, ne_c.replaced_pivots
+    x_c = ne_c.solve(rhs)
+    say("normal equations transport 20x20 dropped pivot")
+    A = gpu_api.SparseMatrix(m_, n_, cp, ri, va)
+    ne = gpu_api.NormalEquations(A)
+    assert ne.factor(d, "fp64")
+    assert ne.replaced_pivots >= 1, ne.replaced_pivots
+    x_g = ne.solve(rhs)
+    at_g = O.spmv(m_, n_, cp, ri, va, x_g, True)
+    at_c = O.spmv(m_, n_, cp, ri, va, x_c, True)
+    np.testing.assert_allclose(at_g, at_c, rtol=0, atol=1e-7 * max(1.0, np.max(np.abs(at_c))))
+    ne.close()
+    A.close()
+
+
 @pytest.mark.parametrize("precision", ["fp64", "fp32"])
 def test_solve_graph_replays_match_eager(gpu_api, monkeypatch, precision):
     """The solve sequences replayed as HIP graphs (the triangular solve pair;
