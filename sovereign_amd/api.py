@@ -182,6 +182,8 @@ class InteriorPointOptions:
     fraction_to_boundary: float = 0.999
     enable_scaling: bool = True
     precision: str = "fp64"  # 'fp64' plain FP64 GPU path, 'mixed' FP32 factors + FP64 refinement
+    # mixed: FP64 factorizations once mu < mixed_switch_mu, or earlier once an
+    # iteration's FP32-preconditioned CG takes more than 8 iterations
     mixed_switch_mu: float = 1e-6
     refine_max: int = 30
     refine_tol: float = 1e-12
