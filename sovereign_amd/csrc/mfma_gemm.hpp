@@ -14,8 +14,8 @@
 // blocks (v_mfma_f64_16x16x4f64 / v_mfma_f32_16x16x4f32). K is staged through
 // LDS in chunks of 32 with a register prefetch of the next chunk; FP32
 // operands are loaded 16 bytes per thread where their layout allows (bulk
-// SYRK of the m = 8192 factorization: 52.9 -> 49.2 us per launch on average
-// against element-wise loads). LDS rows are padded by 16 elements so the four
+// SYRK of the m = 8192 factorization: 51.4 -> 48.3 us per launch on average
+// against element-wise loads, same GPU). LDS rows are padded by 16 elements so the four
 // k-rows one MFMA operand read touches fall in different banks. Workgroups are renumbered so each XCD (round-robin
 // dispatch over 8) works on a contiguous run of tiles, which share operand
 // rows in that XCD's L2.
