@@ -215,6 +215,33 @@ def test_normal_equations_equal_dense_algebra():
         np.testing.assert_allclose(M @ ne.solve(b), b, rtol=0, atol=1e-8 * np.max(np.abs(b)))
 
 
+def test_normal_equations_redundant_row_drops_one_pivot():
+    """A balanced transport problem's equality rows sum to zero in pairs
+    (supply total = demand total), so A has rank m - 1 and A D A^T is
+    singular up to the 1e-12 regularization. With D ~ 1e3 the dependent
+    row's pivot falls under 1e-13 |d0|: the LDL^T's tiny-pivot rule
+    (sparse_ldlt.cpp:165-173) drops exactly one pivot, and consistent
+    right-hand sides (b in range(A)) are still solved: A^T y is unique even
+    though y is not. This is the premise of the GPU dropped-pivot test."""
+    f = O.ipm_form(P.transport_balanced(20, 20))
+    m, n = f["m"], f["n"]
+    A = np.zeros((m, n))
+    for j in range(n):
+        for p in range(f["col_ptr"][j], f["col_ptr"][j + 1]):
+            A[f["row_idx"][p], j] = f["values"][p]
+    assert np.linalg.matrix_rank(A) == m - 1
+    rng = np.random.default_rng(11)
+    d = 1e3 * np.exp(rng.uniform(-1, 1, size=n))
+    rhs = A @ rng.standard_normal(n)
+    ne = O.NormalEquations(m, n, f["col_ptr"], f["row_idx"], f["values"], "sparse")
+    assert ne.factor(d)
+    assert ne.replaced_pivots == 1
+    y = ne.solve(rhs)
+    M = A @ np.diag(d) @ A.T + 1e-12 * np.eye(m)
+    np.testing.assert_allclose(A.T @ y, np.linalg.lstsq(M, rhs, rcond=None)[0] @ A, rtol=0,
+                               atol=1e-7 * np.max(np.abs(A.T @ y)))
+
+
 def test_spmv_equals_dense_product():
     lp = P.random_sparse_lp(90, 130, 5, seed=9)
     A = lp.as_model().dense()
