@@ -790,6 +790,15 @@ T* dev_alloc(sov_batch* h, std::size_t count) {
   return static_cast<T*>(p);
 }
 
+// The pair of timing events of one call, destroyed on every return path.
+struct EventPair {
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  ~EventPair() {
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+  }
+};
+
 // Device buffers owned by one call.
 struct TmpBuffers {
   std::vector<void*> p;
@@ -856,6 +865,10 @@ int sov_batch_solve(sov_batch* h, int32_t count, const int32_t* matrix_id, const
   for (int k = 0; k < count; ++k) {
     const int id = matrix_id[k];
     if (id < 0 || id >= static_cast<int>(h->mats.size())) return SOV_ERR_ARG;
+    if (b_off[k] < 0 || x_off[k] < 0) {
+      sov::set_error("sov_batch_solve: negative b_off / x_off");
+      return SOV_ERR_ARG;
+    }
     const BatchMatrix& mt = h->mats[id];
     mmax = std::max(mmax, mt.m);
     descs[k] = {id, 0, b_off[k], x_off[k], ws_total};
@@ -891,9 +904,10 @@ int sov_batch_solve(sov_batch* h, int32_t count, const int32_t* matrix_id, const
   SOV_HIP_TRY(hipGetDevice(&dev));
   SOV_HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
   const unsigned grid = static_cast<unsigned>(std::min(count, std::max(cus, 1)));
-  hipEvent_t e0, e1;
-  SOV_HIP_TRY(hipEventCreate(&e0));
-  SOV_HIP_TRY(hipEventCreate(&e1));
+  EventPair ev;
+  SOV_HIP_TRY(hipEventCreate(&ev.e0));
+  SOV_HIP_TRY(hipEventCreate(&ev.e1));
+  hipEvent_t e0 = ev.e0, e1 = ev.e1;
   (void)hipEventRecord(e0, s);
   // SOVEREIGN_BATCH_TRACE=k records the iterations of LP k (stderr).
   const char* tenv = std::getenv("SOVEREIGN_BATCH_TRACE");
@@ -909,8 +923,6 @@ int sov_batch_solve(sov_batch* h, int32_t count, const int32_t* matrix_id, const
   const hipError_t le = hipGetLastError();
   (void)hipEventRecord(e1, s);
   if (le != hipSuccess) {
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
     sov::set_error(std::string("batch_ipm_kernel launch: ") + hipGetErrorString(le));
     return SOV_ERR_DEVICE;
   }
@@ -932,17 +944,7 @@ int sov_batch_solve(sov_batch* h, int32_t count, const int32_t* matrix_id, const
                    static_cast<int>(q[8]), static_cast<int>(q[15]), q[9], q[10], q[11], q[12], q[13], q[14]);
     }
   }
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
-  for (int k = 0; k < count; ++k) {
-    res[k].status = host[k].status;
-    res[k].iterations = host[k].iterations;
-    res[k].pcg_iterations = host[k].pcg_iterations;
-    res[k].objective = host[k].objective;
-    res[k].duality_gap = host[k].duality_gap;
-    res[k].primal_residual = host[k].primal_residual;
-    res[k].dual_residual = host[k].dual_residual;
-  }
+  for (int k = 0; k < count; ++k) res[k] = host[k];
   sov::count_operation();
   return SOV_OK;
 }

@@ -69,6 +69,60 @@ def test_strong_branch_children_match_oracle(gpu_api):
         assert rel(a["obj_down"], b["obj_down"]) <= 1e-8 and rel(a["obj_up"], b["obj_up"]) <= 1e-8
 
 
+def test_strong_branch_full_size_root(gpu_api):
+    """The strong-branching workload at its real size: the root of the
+    n=500, m=200 knapsack (seed 26119), 64 candidates x 2 = 128 child LPs of
+    200 rows (700 columns with the slacks) in one persistent launch. The
+    root point and all 128 child objectives come from the reference's own
+    CPU interior-point solver (tests/golden/c3_children.json); every child
+    must match to 1e-8 relative, and the choice must equal the per-LP GPU
+    path's."""
+    import json
+    import os
+
+    with open(os.path.join(os.path.dirname(__file__), "golden", "c3_children.json")) as fh:
+        gold = json.load(fh)
+    inst = gold["instance"]
+    model = P.multi_knapsack(inst["n"], inst["m"], seed=inst["seed"])
+    is_int = np.ones(model.n_vars, dtype=np.int8)
+    x_root = np.array([float(v) for v in gold["root"]["primal"]])
+    parent = float(gold["root"]["objective"])
+    say("strong branch mkp 500x200 root, 128 children batched")
+    gpu_api.reset_gpu_operations()
+    cands, best, kms = MI.strong_branch(model, is_int, x_root, parent, max_candidates=64)
+    ops = gpu_api.gpu_operations()
+    say(f"  kernel {kms:.1f} ms, {ops} GPU operations, best {best}")
+    assert ops == 1, "every child must finish inside the one batched launch"
+    assert [c["var"] for c in cands] == gold["candidates"]
+    worst = 0.0
+    for c in cands:
+        for side in ("down", "up"):
+            ref = gold["children"][f"{c['var']}:{side}"]
+            assert c[f"status_{side}"] in (0, 2)
+            assert c[f"status_{side}"] == ref["status"], (c["var"], side)
+            if ref["status"] == 0:
+                d = rel(c[f"obj_{side}"], float(ref["objective"]))
+                worst = max(worst, d)
+                assert d <= 1e-8, (c["var"], side, c[f"obj_{side}"], ref["objective"])
+    say(f"  worst child objective rel diff vs reference {worst:.2e}")
+    # the reference's score and first-best choice, from the reference's objectives
+    def deg(st, obj):
+        return 1e6 if st == 2 else max(0.0, abs(obj - parent))
+    scores = []
+    for v in gold["candidates"]:
+        d = gold["children"][f"{v}:down"]
+        u = gold["children"][f"{v}:up"]
+        dd, du = deg(d["status"], float(d["objective"])), deg(u["status"], float(u["objective"]))
+        scores.append(min(dd, du) + 1e-4 * max(dd, du))
+    ref_best = int(np.argmax(scores))
+    top = sorted(scores)[-2:]
+    if top[1] - top[0] > 1e-6 * max(1.0, top[1]):  # a clear winner: the choice is pinned
+        assert best == ref_best
+    say("strong branch mkp 500x200 root, 128 children per-LP")
+    _, best_p, _ = MI.strong_branch(model, is_int, x_root, parent, max_candidates=64, child_solver=MI.CHILD_PER_LP)
+    assert best_p == best
+
+
 def test_batch_uneven_sizes_match_oracle(gpu_api):
     """LPs of two different shapes in one launch, each checked against the
     oracle on the same LP written as a general model."""
