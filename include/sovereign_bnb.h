@@ -50,35 +50,64 @@ int sov_strong_branch(const sov_lp_model* node, const double* node_lb, const dou
 typedef void (*sov_candidate_order_fn)(int32_t n_vars, const double* x, const double* lb,
                                        const double* ub, double* key, void* user);
 
+/* Branching rules of BranchRule (solver/milp/include/sovereign/branch_and_bound.hpp:7-16). */
+#define SOV_BRANCH_MOST_FRACTIONAL 0 /* pick_most_fractional (branch_and_bound.cpp:144-159) */
+#define SOV_BRANCH_RELIABILITY 1     /* StrongBranching = pick_reliability_branch (:799-899), the default */
+#define SOV_BRANCH_PSEUDOCOST 2      /* pick_pseudo_cost (:161-189) */
+#define SOV_BRANCH_FULL_STRONG 3     /* pick_strong_branch (:726-793) */
+
+/* BranchAndBoundOptions (branch_and_bound.hpp:18-59) for the parts built
+ * here; cuts, heuristics and presolve are not. */
 typedef struct {
-  int64_t max_nodes;
-  int32_t strong_branch_candidates; /* k candidates strong-branched per node */
+  int64_t max_nodes;                /* 100000 */
+  int32_t strong_branch_candidates; /* FullStrong: k candidates per node (the reference's default is 4) */
   int32_t child_solver;             /* SOV_CHILD_BATCHED or SOV_CHILD_PER_LP */
   double mip_gap;                   /* 1e-6 relative (branch_and_bound.hpp:23) */
   double integer_tol;               /* 1e-6 */
   double time_limit_seconds;        /* 0 = none */
-  sov_candidate_order_fn candidate_order; /* NULL: most fractional first */
+  sov_candidate_order_fn candidate_order; /* FullStrong only. NULL: closest fraction to 1/2 first */
   void* candidate_order_user;
+  int32_t branch_rule;           /* SOV_BRANCH_*, default SOV_BRANCH_RELIABILITY */
+  int32_t reliability_threshold; /* 4: observations per direction before a pseudocost is trusted */
+  int32_t max_strong_per_node;   /* 8: reliability branching's strong-branch budget per node */
+  int32_t plunging;              /* 1: continue with the preferred child before the best-bound queue */
 } sov_bnb_options;
 
 typedef struct {
-  int32_t status; /* 0 optimal, 1 feasible (limit hit), 2 infeasible, 5 limit without incumbent, 6 numerical */
+  int32_t status; /* 0 optimal, 1 feasible (limit hit or a node LP failed), 2 infeasible,
+                   * 5 limit without incumbent, 6 numerical (no incumbent and a node LP failed) */
   int32_t pad;
   double objective;
   double root_bound;
   int64_t nodes;
-  int64_t lp_solves;
+  int64_t lp_solves;           /* node LPs + strong-branching child LPs + incumbent polishes */
   double seconds;
   double strong_branch_seconds;
-  double kernel_ms;
+  double kernel_ms;            /* batched launches' kernel time */
+  int64_t node_lp_solves;      /* node LPs actually solved (a node whose LP strong branching already
+                                * solved reuses that solution) */
+  int64_t strong_lp_solves;    /* child LPs solved for strong branching */
+  int64_t strong_lp_used;      /* of those, the ones the reference's sequential rule would have solved */
+  double node_lp_seconds;
+  int32_t max_depth;
+  int32_t node_lp_failures;    /* subtrees dropped because their LP did not solve */
 } sov_bnb_result;
 
 void sov_bnb_default_options(sov_bnb_options* opt);
 
-/* Best-bound branch-and-bound with strong branching
- * (BranchAndBoundSolver::solve, branch_and_bound.cpp:906-1523, without cuts,
- * heuristics or node presolve). is_int marks integer variables; x_best
- * (n_vars, may be NULL) receives the incumbent. */
+/* BranchAndBoundSolver::solve (branch_and_bound.cpp:906-1523) without cuts,
+ * heuristics or node presolve: best-bound queue with plunging, the node LP
+ * solved when the node is popped (reusing a child LP that strong branching
+ * already solved), pseudocost learning from node LPs (:1159-1170) and from
+ * strong branching, integrality checked before bound pruning (:1317), and
+ * the branching rule of bopt->branch_rule. Reliability branching's strong
+ * branches run speculatively as one batched launch of the first
+ * max_strong_per_node unreliable candidates; the reference's sequential
+ * look-ahead rule is then replayed over their results, so the decision and
+ * the pseudocost updates are the reference's. is_int marks integer
+ * variables; x_best (n_vars, may be NULL) receives the incumbent. Variables
+ * with lb <= -1e29 cannot be shifted into the batched child form; for such
+ * models every LP goes through the per-LP path. */
 int sov_bnb_solve(const sov_lp_model* milp, const int8_t* is_int, const sov_bnb_options* bopt,
                   const sov_ipm_options* opt, sov_bnb_result* res, double* x_best);
 

@@ -16,6 +16,7 @@ LIB_PATH = os.path.join(_HERE, "libsovereign_hip.so")
 
 SOV_OK = 0
 SOV_SINGULAR = 1
+SOV_ERR_ARG = -1
 SOV_PREC_FP64 = 0
 SOV_PREC_FP32 = 1
 
@@ -105,6 +106,8 @@ class BnbOptions(C.Structure):
         ("max_nodes", C.c_int64), ("strong_branch_candidates", C.c_int32), ("child_solver", C.c_int32),
         ("mip_gap", C.c_double), ("integer_tol", C.c_double), ("time_limit_seconds", C.c_double),
         ("candidate_order", CANDIDATE_ORDER_FN), ("candidate_order_user", C.c_void_p),
+        ("branch_rule", C.c_int32), ("reliability_threshold", C.c_int32), ("max_strong_per_node", C.c_int32),
+        ("plunging", C.c_int32),
     ]
 
 
@@ -112,7 +115,9 @@ class BnbResult(C.Structure):
     _fields_ = [
         ("status", C.c_int32), ("pad", C.c_int32), ("objective", C.c_double), ("root_bound", C.c_double),
         ("nodes", C.c_int64), ("lp_solves", C.c_int64), ("seconds", C.c_double),
-        ("strong_branch_seconds", C.c_double), ("kernel_ms", C.c_double),
+        ("strong_branch_seconds", C.c_double), ("kernel_ms", C.c_double), ("node_lp_solves", C.c_int64),
+        ("strong_lp_solves", C.c_int64), ("strong_lp_used", C.c_int64), ("node_lp_seconds", C.c_double),
+        ("max_depth", C.c_int32), ("node_lp_failures", C.c_int32),
     ]
 
     def as_dict(self):

@@ -22,6 +22,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
+#include <functional>
 #include <limits>
 #include <memory>
 #include <queue>
@@ -136,6 +137,107 @@ double model_objective(const DenseForm& F, double scaled_min_obj) {
 struct ChildSpec {
   std::vector<double> lb, ub;
   bool empty = false;
+};
+
+// Node presolve: proves a node LP infeasible from its rows' activity ranges
+// under the node's bounds, with activity-based bound propagation between
+// passes (the infeasibility tests of Presolver, solver/presolve/src/
+// presolve.cpp:186-246 and 560-660, with its margin 1e-6 (1 + |scale|) and
+// |candidate| <= 1e9). The node's branch-and-bound descends into LPs that an
+// interior point cannot certify infeasible (it only stalls); this test
+// catches those that bound propagation proves empty. The propagated bounds
+// serve only the test (no integer rounding, so what it proves holds for the
+// LP relaxation); the LP itself is solved on the node's own bounds.
+class NodePresolve {
+ public:
+  explicit NodePresolve(const sov_lp_model& mdl) : m_(mdl.n_rows), n_(mdl.n_vars), sense_(mdl.row_sense, mdl.row_sense + mdl.n_rows), rhs_(mdl.rhs, mdl.rhs + mdl.n_rows) {
+    ptr_.assign(static_cast<std::size_t>(m_) + 1, 0);
+    for (int j = 0; j < n_; ++j)
+      for (int p = mdl.col_ptr[j]; p < mdl.col_ptr[j + 1]; ++p) ++ptr_[mdl.row_idx[p] + 1];
+    for (int r = 0; r < m_; ++r) ptr_[r + 1] += ptr_[r];
+    col_.resize(static_cast<std::size_t>(ptr_[m_]));
+    val_.resize(col_.size());
+    std::vector<int> fill(ptr_.begin(), ptr_.end() - 1);
+    for (int j = 0; j < n_; ++j)
+      for (int p = mdl.col_ptr[j]; p < mdl.col_ptr[j + 1]; ++p) {
+        const int q = fill[mdl.row_idx[p]]++;
+        col_[q] = j;
+        val_[q] = mdl.values[p];
+      }
+  }
+
+  // lb, ub: at least n_vars entries (structural bounds first).
+  bool infeasible(std::vector<double> lb, std::vector<double> ub) const {
+    constexpr double kInf = 1e29;
+    auto margin = [](double scale) { return 1e-6 * (1.0 + std::abs(scale)); };
+    for (int j = 0; j < n_; ++j)
+      if (lb[j] - ub[j] > margin(std::max(std::abs(lb[j]), std::abs(ub[j])))) return true;
+    for (int pass = 0; pass < 8; ++pass) {
+      bool changed = false;
+      for (int r = 0; r < m_; ++r) {
+        const char sn = sense_[r];
+        const bool le = sn == 'L' || sn == 'l' || sn == 'E' || sn == 'e';
+        const bool ge = sn == 'G' || sn == 'g' || sn == 'E' || sn == 'e';
+        double mn = 0.0, mx = 0.0, mag = 0.0;
+        int mn_inf = 0, mx_inf = 0;
+        for (int p = ptr_[r]; p < ptr_[r + 1]; ++p) {
+          const double a = val_[p];
+          const int j = col_[p];
+          const double lo = a > 0 ? lb[j] : ub[j], hi = a > 0 ? ub[j] : lb[j];
+          if (std::abs(lo) >= kInf) ++mn_inf; else mn += a * lo;
+          if (std::abs(hi) >= kInf) ++mx_inf; else mx += a * hi;
+          mag += (std::abs(lo) >= kInf ? 0.0 : std::abs(a * lo)) + (std::abs(hi) >= kInf ? 0.0 : std::abs(a * hi));
+        }
+        const double mg = margin(mag);
+        if (le && mn_inf == 0 && mn > rhs_[r] + mg) return true;
+        if (ge && mx_inf == 0 && mx < rhs_[r] - mg) return true;
+        // a_j x_j <= rhs - (min activity of the rest), >= rhs - (max of the rest)
+        for (int p = ptr_[r]; p < ptr_[r + 1]; ++p) {
+          const double a = val_[p];
+          const int j = col_[p];
+          const double lo = a > 0 ? lb[j] : ub[j], hi = a > 0 ? ub[j] : lb[j];
+          const bool lo_inf = std::abs(lo) >= kInf, hi_inf = std::abs(hi) >= kInf;
+          if (le && mn_inf - (lo_inf ? 1 : 0) == 0) {
+            const double cand = (rhs_[r] - (mn - (lo_inf ? 0.0 : a * lo))) / a;
+            if (std::abs(cand) <= 1e9) {
+              double& bnd = a > 0 ? ub[j] : lb[j];
+              if (a > 0 ? cand < bnd - margin(cand) : cand > bnd + margin(cand)) {
+                bnd = cand;
+                changed = true;
+              }
+            }
+          }
+          if (ge && mx_inf - (hi_inf ? 1 : 0) == 0) {
+            const double cand = (rhs_[r] - (mx - (hi_inf ? 0.0 : a * hi))) / a;
+            if (std::abs(cand) <= 1e9) {
+              double& bnd = a > 0 ? lb[j] : ub[j];
+              if (a > 0 ? cand > bnd + margin(cand) : cand < bnd - margin(cand)) {
+                bnd = cand;
+                changed = true;
+              }
+            }
+          }
+          if (lb[j] - ub[j] > margin(std::max(std::abs(lb[j]), std::abs(ub[j])))) return true;
+        }
+      }
+      if (!changed) break;
+    }
+    return false;
+  }
+
+  // Marks the children whose LP this test proves infeasible.
+  template <typename Spec>
+  void mark(std::vector<Spec>& kids) const {
+    for (Spec& k : kids)
+      if (!k.empty && infeasible(k.lb, k.ub)) k.empty = true;
+  }
+
+ private:
+  int m_, n_;
+  std::vector<char> sense_;
+  std::vector<double> rhs_;
+  std::vector<int> ptr_, col_;
+  std::vector<double> val_;
 };
 
 struct ChildOut {
@@ -286,6 +388,230 @@ double degradation(const ChildOut& c, double parent) {
   return std::max(0.0, std::abs(obj - parent));
 }
 
+// A variable with lb <= -1e29 has no finite shift into the batched child form
+// 0 <= x <= u; such models go through the per-LP path.
+bool batched_form_ok(const double* lb, int nv) {
+  for (int j = 0; j < nv; ++j)
+    if (lb[j] <= -1e29) return false;
+  return true;
+}
+
+// PseudoCostStats (branch_and_bound.cpp:74-79), indexed by variable. A
+// variable is "known" to the reference once either count is positive: every
+// site that creates its map entry also increments a count.
+struct PseudoCost {
+  double down_sum = 0.0, up_sum = 0.0;
+  int down_count = 0, up_count = 0;
+};
+
+// product_score (branch_and_bound.cpp:795-797).
+double product_score(double down_gain, double up_gain) {
+  return std::max(down_gain, 1e-6) * std::max(up_gain, 1e-6);
+}
+
+// The branching decision of one node, with the two children's LP results
+// when strong branching solved them (the node LP of that child is then not
+// solved again: it is the same LP).
+struct Decision {
+  int var = -1;
+  bool have_children = false;
+  ChildOut down, up;
+};
+
+// Solves the down/up children of `vars` at a node as one set (one batched
+// launch, or one per-LP solve each); out[2q], out[2q+1] are var q's down, up.
+using PairSolver = std::function<int(const std::vector<int>& vars, std::vector<ChildOut>& out)>;
+
+// pick_most_fractional (branch_and_bound.cpp:144-159): the largest
+// fractional part, the first such variable on ties.
+int pick_most_fractional(const std::vector<int>& cands, const double* x) {
+  int best = -1;
+  double best_score = -1.0;
+  for (int i : cands) {
+    const double score = frac_part(x[i]);
+    if (score > best_score) {
+      best_score = score;
+      best = i;
+    }
+  }
+  return best;
+}
+
+// pick_pseudo_cost (branch_and_bound.cpp:161-189).
+int pick_pseudo_cost(const std::vector<int>& cands, const double* x, const std::vector<PseudoCost>& st) {
+  int best = -1;
+  double best_score = -1.0;
+  for (int i : cands) {
+    const double fdown = x[i] - std::floor(x[i]), fup = std::ceil(x[i]) - x[i];
+    double down_est = fdown, up_est = fup;
+    if (st[i].down_count > 0) down_est = fdown * (st[i].down_sum / st[i].down_count);
+    if (st[i].up_count > 0) up_est = fup * (st[i].up_sum / st[i].up_count);
+    const double score = std::min(down_est, up_est) + 1e-4 * std::max(down_est, up_est);
+    if (score > best_score) {
+      best_score = score;
+      best = i;
+    }
+  }
+  return best >= 0 ? best : pick_most_fractional(cands, x);
+}
+
+// pick_strong_branch (branch_and_bound.cpp:726-793) with every candidate's
+// two children in one set. `order` (may be empty) is a ranking key per
+// variable applied, stably, after the closeness order (the learned
+// surrogate); the first k candidates are branched.
+Decision pick_full_strong(const std::vector<int>& frac, const double* x, double parent, double tol, int k,
+                          const std::vector<double>& order, std::vector<PseudoCost>& st, const PairSolver& solve,
+                          int* rc, long long* solved) {
+  Decision d;
+  std::vector<int> cands = frac;
+  std::sort(cands.begin(), cands.end(),
+            [&](int a, int b) { return std::abs(frac_part(x[a]) - 0.5) < std::abs(frac_part(x[b]) - 0.5); });
+  if (!order.empty())
+    std::stable_sort(cands.begin(), cands.end(), [&](int a, int b) { return order[a] > order[b]; });
+  if (static_cast<int>(cands.size()) > k) cands.resize(static_cast<std::size_t>(std::max(k, 0)));
+  if (cands.empty()) {
+    d.var = pick_most_fractional(frac, x);
+    return d;
+  }
+  std::vector<ChildOut> outs;
+  if ((*rc = solve(cands, outs)) != SOV_OK) return d;
+  *solved += 2 * static_cast<long long>(cands.size());
+  int best = -1;
+  double best_score = -1.0;
+  for (std::size_t q = 0; q < cands.size(); ++q) {
+    const int i = cands[q];
+    const ChildOut& dn = outs[2 * q];
+    const ChildOut& up = outs[2 * q + 1];
+    const double down_obj = dn.status == 0 ? dn.objective : parent;
+    const double up_obj = up.status == 0 ? up.objective : parent;
+    const double fdown = x[i] - std::floor(x[i]), fup = std::ceil(x[i]) - x[i];
+    if (fdown > tol) {
+      st[i].down_sum += std::abs(down_obj - parent) / fdown;
+      ++st[i].down_count;
+    }
+    if (fup > tol) {
+      st[i].up_sum += std::abs(up_obj - parent) / fup;
+      ++st[i].up_count;
+    }
+    const double dd = degradation(dn, parent), du = degradation(up, parent);
+    const double score = std::min(dd, du) + 1e-4 * std::max(dd, du);
+    if (score > best_score) {
+      best_score = score;
+      best = static_cast<int>(q);
+    }
+  }
+  if (best < 0) {
+    d.var = pick_most_fractional(frac, x);
+    return d;
+  }
+  d.var = cands[best];
+  d.have_children = true;
+  d.down = std::move(outs[2 * best]);
+  d.up = std::move(outs[2 * best + 1]);
+  return d;
+}
+
+// pick_reliability_branch (branch_and_bound.cpp:799-899). The reference
+// strong-branches unreliable candidates one at a time, in pseudocost order,
+// until max_strong are done or kLookahead in a row fail to improve the best
+// score. Every candidate that loop can reach is among the first max_strong
+// unreliable ones, so their children are solved together in one set first;
+// the sequential rule is then replayed over those results, and only what it
+// reaches updates the pseudocosts or decides, exactly as in the reference.
+Decision pick_reliability(const std::vector<int>& frac, const double* x, double parent, double tol, int reliability,
+                          int max_strong, std::vector<PseudoCost>& st, const PairSolver& solve, int* rc,
+                          long long* solved, long long* used) {
+  Decision d;
+  if (frac.empty()) return d;
+  double down_sum = 0.0, up_sum = 0.0;
+  int down_n = 0, up_n = 0;
+  for (const PseudoCost& p : st) {
+    if (p.down_count > 0) {
+      down_sum += p.down_sum / p.down_count;
+      ++down_n;
+    }
+    if (p.up_count > 0) {
+      up_sum += p.up_sum / p.up_count;
+      ++up_n;
+    }
+  }
+  const double avg_down = down_n > 0 ? down_sum / down_n : 1.0;
+  const double avg_up = up_n > 0 ? up_sum / up_n : 1.0;
+  struct Scored {
+    int var;
+    double score;
+    bool reliable;
+  };
+  std::vector<Scored> scored;
+  scored.reserve(frac.size());
+  for (int i : frac) {
+    const double fdown = x[i] - std::floor(x[i]), fup = std::ceil(x[i]) - x[i];
+    const PseudoCost& p = st[i];
+    const bool known = p.down_count > 0 || p.up_count > 0;
+    const double unit_down = p.down_count > 0 ? p.down_sum / p.down_count : avg_down;
+    const double unit_up = p.up_count > 0 ? p.up_sum / p.up_count : avg_up;
+    const bool reliable = known && std::min(p.down_count, p.up_count) >= reliability;
+    scored.push_back({i, product_score(fdown * unit_down, fup * unit_up), reliable});
+  }
+  std::sort(scored.begin(), scored.end(), [](const Scored& a, const Scored& b) { return a.score > b.score; });
+  std::vector<int> spec;  // the candidates the sequential loop can reach, in its order
+  for (const Scored& s : scored)
+    if (!s.reliable && static_cast<int>(spec.size()) < max_strong) spec.push_back(s.var);
+  std::vector<ChildOut> outs;
+  if (!spec.empty()) {
+    if ((*rc = solve(spec, outs)) != SOV_OK) return d;
+    *solved += 2 * static_cast<long long>(spec.size());
+  }
+  constexpr int kLookahead = 4;
+  int best = scored.front().var, best_q = -1;
+  double best_score = -1.0;
+  int strong_done = 0, no_improve = 0;
+  for (const Scored& s : scored) {
+    double score = s.score;
+    int q = -1;
+    if (!s.reliable && strong_done < max_strong && no_improve < kLookahead) {
+      q = strong_done++;  // spec[q] == s.var: the q-th unreliable candidate in this order
+      *used += 2;
+      const ChildOut& dn = outs[2 * q];
+      const ChildOut& up = outs[2 * q + 1];
+      const bool down_inf = dn.status == 2, up_inf = up.status == 2;
+      if (down_inf && up_inf) {  // both children die; any choice prunes the node
+        d.var = s.var;
+        d.have_children = true;
+        d.down = dn;
+        d.up = up;
+        return d;
+      }
+      const double fdown = x[s.var] - std::floor(x[s.var]), fup = std::ceil(x[s.var]) - x[s.var];
+      double down_deg = down_inf ? kInfeasibleDegradation : 0.0, up_deg = up_inf ? kInfeasibleDegradation : 0.0;
+      if (!down_inf && dn.status == 0) {
+        down_deg = std::abs(dn.objective - parent);
+        st[s.var].down_sum += down_deg / std::max(fdown, tol);
+        ++st[s.var].down_count;
+      }
+      if (!up_inf && up.status == 0) {
+        up_deg = std::abs(up.objective - parent);
+        st[s.var].up_sum += up_deg / std::max(fup, tol);
+        ++st[s.var].up_count;
+      }
+      score = product_score(down_deg, up_deg);
+      no_improve = score > best_score ? 0 : no_improve + 1;
+    }
+    if (score > best_score) {
+      best_score = score;
+      best = s.var;
+      best_q = q;
+    }
+  }
+  d.var = best;
+  if (best_q >= 0) {
+    d.have_children = true;
+    d.down = std::move(outs[2 * best_q]);
+    d.up = std::move(outs[2 * best_q + 1]);
+  }
+  return d;
+}
+
 }  // namespace
 
 extern "C" {
@@ -299,6 +625,10 @@ void sov_bnb_default_options(sov_bnb_options* o) {
   o->time_limit_seconds = 0.0;
   o->candidate_order = nullptr;
   o->candidate_order_user = nullptr;
+  o->branch_rule = SOV_BRANCH_RELIABILITY;
+  o->reliability_threshold = 4;
+  o->max_strong_per_node = 8;
+  o->plunging = 1;
 }
 
 int sov_strong_branch(const sov_lp_model* node, const double* node_lb, const double* node_ub, const int8_t* is_int,
@@ -313,12 +643,17 @@ int sov_strong_branch(const sov_lp_model* node, const double* node_lb, const dou
   *ncand_out = static_cast<int32_t>(cand.size());
   if (cand.empty()) return SOV_OK;
   const DenseForm F = dense_form(*node, node_lb, node_ub);
+  if (child_solver == SOV_CHILD_BATCHED && F.m > 272) {
+    sov::set_error("sov_strong_branch: batched child LPs need at most 272 rows");
+    return SOV_ERR_ARG;
+  }
   std::vector<double> lb(node_lb, node_lb + node->n_vars), ub(node_ub, node_ub + node->n_vars);
   std::vector<ChildSpec> kids;
   children_specs(lb, ub, cand, x_node, F.n - F.nv, kids);
+  NodePresolve(*node).mark(kids);
   std::vector<ChildOut> res;
   int rc;
-  if (child_solver == SOV_CHILD_BATCHED) {
+  if (child_solver == SOV_CHILD_BATCHED && batched_form_ok(node_lb, node->n_vars)) {
     sov_batch* batch = nullptr;
     if ((rc = sov_batch_create(&batch)) != SOV_OK) return rc;
     int32_t mid = 0;
@@ -357,19 +692,24 @@ int sov_bnb_solve(const sov_lp_model* milp, const int8_t* is_int, const sov_bnb_
   if (!milp || !is_int || !bo || !opt || !res) return SOV_ERR_ARG;
   *res = sov_bnb_result{};
   res->status = 7;
+  if (bo->branch_rule < SOV_BRANCH_MOST_FRACTIONAL || bo->branch_rule > SOV_BRANCH_FULL_STRONG) {
+    sov::set_error("sov_bnb_solve: unknown branch_rule");
+    return SOV_ERR_ARG;
+  }
   if (!sov::device_ready()) return SOV_ERR_NODEV;
   const auto t0 = Clock::now();
   const int nv = milp->n_vars;
-  std::vector<double> lb0(milp->lb, milp->lb + nv), ub0(milp->ub, milp->ub + nv);
+  const std::vector<double> lb0(milp->lb, milp->lb + nv), ub0(milp->ub, milp->ub + nv);
   const DenseForm F = dense_form(*milp, lb0.data(), ub0.data());
   const int nslack = F.n - F.nv;
-  if (F.m > 272 && bo->child_solver == SOV_CHILD_BATCHED) {
+  const bool batched = bo->child_solver == SOV_CHILD_BATCHED && batched_form_ok(lb0.data(), nv);
+  if (batched && F.m > 272) {
     sov::set_error("sov_bnb_solve: batched child LPs need at most 272 rows");
     return SOV_ERR_ARG;
   }
   std::unique_ptr<sov_batch, int (*)(sov_batch*)> batch(nullptr, sov_batch_free);
   int32_t mat_id = 0;
-  if (bo->child_solver == SOV_CHILD_BATCHED) {
+  if (batched) {
     sov_batch* b = nullptr;
     int rc = sov_batch_create(&b);
     if (rc != SOV_OK) return rc;
@@ -379,114 +719,235 @@ int sov_bnb_solve(const sov_lp_model* milp, const int8_t* is_int, const sov_bnb_
   }
   double kernel_ms = 0.0;
   int lp_solves = 0;
-  double sb_seconds = 0.0;
-  auto solve_set = [&](const std::vector<ChildSpec>& kids, std::vector<ChildOut>& outv) {
-    if (bo->child_solver == SOV_CHILD_BATCHED) {
+  const NodePresolve presolve(*milp);
+  auto solve_set = [&](std::vector<ChildSpec> kids, std::vector<ChildOut>& outv) {
+    presolve.mark(kids);
+    if (batched) {
       const int rc = solve_children(batch.get(), mat_id, F, kids, *opt, outv, &kernel_ms, &lp_solves);
       return rc != SOV_OK ? rc : finish_unconverged(*milp, kids, F, *opt, outv, &lp_solves);
     }
     return solve_children_one_by_one(*milp, kids, F, *opt, outv, &lp_solves);
   };
-  // Minimisation view: bounds and incumbent in F.sense * objective.
-  struct Node {
-    double bound;  // min-sense LP bound
-    long long id;
-    std::vector<double> lb, ub, x;
+  auto node_spec = [&](const std::vector<double>& lb, const std::vector<double>& ub) {
+    ChildSpec s;
+    s.lb = lb;
+    s.ub = ub;
+    s.lb.resize(static_cast<std::size_t>(F.n), 0.0);
+    s.ub.resize(static_cast<std::size_t>(F.n), 1e30);
+    return s;
   };
-  auto worse = [](const Node& a, const Node& b) { return a.bound != b.bound ? a.bound > b.bound : a.id > b.id; };
-  std::priority_queue<Node, std::vector<Node>, decltype(worse)> open(worse);
-  long long next_id = 0;
+  bool any_continuous = false;
+  for (int j = 0; j < nv; ++j) any_continuous = any_continuous || !is_int[j];
+
+  // Open nodes carry their branching bounds on top of the root's
+  // (SearchNode, branch_and_bound.cpp:58-71). Bounds and objectives are kept
+  // in the minimisation sense (F.sense * model objective).
+  struct BoundChange {
+    int var;
+    double lower, upper;
+  };
+  struct OpenNode {
+    double bound = 0.0;
+    int depth = 0;
+    long long id = 0;
+    std::vector<BoundChange> bounds;
+    int branch_var = -1;
+    bool branch_up = false;
+    double branch_distance = 0.0, parent_obj = 0.0;
+    std::shared_ptr<const ChildOut> solved;  // the LP, when strong branching solved it
+  };
+  auto worse = [](const OpenNode& a, const OpenNode& b) { return a.bound != b.bound ? a.bound > b.bound : a.id > b.id; };
+  std::priority_queue<OpenNode, std::vector<OpenNode>, decltype(worse)> open(worse);
+  std::vector<OpenNode> plunge;  // the preferred child, popped next (branch_and_bound.cpp:1014-1048)
+  long long next_id = 1;
   {
-    ChildSpec root;
-    root.lb = lb0;
-    root.ub = ub0;
-    root.lb.resize(static_cast<std::size_t>(F.n), 0.0);
-    root.ub.resize(static_cast<std::size_t>(F.n), 1e30);
-    std::vector<ChildOut> r;
-    const int rc = solve_set({root}, r);
-    if (rc != SOV_OK) return rc;
-    if (r[0].status == 2) {
-      res->status = 2;
-      return SOV_OK;
-    }
-    if (r[0].status != 0) {
-      res->status = 6;
-      return SOV_OK;
-    }
-    res->root_bound = r[0].objective;
-    open.push({F.sense * r[0].objective, next_id++, lb0, ub0, r[0].x});
+    OpenNode root;
+    root.bound = -std::numeric_limits<double>::infinity();
+    root.id = next_id++;
+    open.push(std::move(root));
   }
-  double incumbent = std::numeric_limits<double>::infinity();  // min sense
+  auto rel_gap = [](double bound, double inc) { return (inc - bound) / std::max(1.0, std::abs(inc)); };
+  std::vector<PseudoCost> pseudo(static_cast<std::size_t>(nv));
+  bool has_inc = false, any_error = false, limit_hit = false, have_root = false;
+  double incumbent = 0.0;  // min sense
   std::vector<double> best_x;
-  long long nodes = 0;
-  bool limit_hit = false;
-  while (!open.empty()) {
-    if (nodes >= bo->max_nodes ||
-        (bo->time_limit_seconds > 0 &&
-         std::chrono::duration<double>(Clock::now() - t0).count() > bo->time_limit_seconds)) {
+  long long nodes = 0, node_lps = 0, strong_lps = 0, strong_used = 0;
+  double sb_seconds = 0.0, lp_seconds = 0.0;
+  int max_depth = 0, failures = 0;
+
+  auto offer = [&](std::vector<double> x, double obj_min) -> int {  // offer_incumbent + polish_incumbent
+    if (has_inc && !(obj_min < incumbent - 1e-12)) return SOV_OK;
+    if (any_continuous) {
+      std::vector<double> lb = lb0, ub = ub0;
+      for (int j = 0; j < nv; ++j)
+        if (is_int[j]) lb[j] = ub[j] = std::round(x[j]);
+      std::vector<ChildOut> r;
+      const int rc = solve_set({node_spec(lb, ub)}, r);
+      if (rc != SOV_OK) return rc;
+      if (r[0].status == 0) {
+        for (int j = 0; j < nv; ++j) x[j] = is_int[j] ? lb[j] : r[0].x[j];
+        obj_min = F.sense * r[0].objective;
+      }
+      if (has_inc && !(obj_min < incumbent - 1e-12)) return SOV_OK;
+    }
+    has_inc = true;
+    incumbent = obj_min;
+    best_x = std::move(x);
+    return SOV_OK;
+  };
+
+  while (!(plunge.empty() && open.empty())) {
+    if (nodes >= bo->max_nodes || (bo->time_limit_seconds > 0 &&
+                                   std::chrono::duration<double>(Clock::now() - t0).count() >= bo->time_limit_seconds)) {
       limit_hit = true;
       break;
     }
-    Node node = open.top();
-    open.pop();
-    const double tol = bo->mip_gap * std::max(1.0, std::abs(incumbent));
-    if (node.bound >= incumbent - tol) continue;  // pruned by bound
+    OpenNode node;
+    if (!plunge.empty()) {
+      node = std::move(plunge.back());
+      plunge.pop_back();
+    } else {
+      node = open.top();
+      open.pop();
+    }
     ++nodes;
-    std::vector<int> cand = fractional_candidates(is_int, node.x.data(), nv, bo->integer_tol);
-    if (cand.empty()) {  // integral: new incumbent
-      incumbent = node.bound;
-      best_x = node.x;
-      for (int j = 0; j < nv; ++j)
-        if (is_int[j]) best_x[j] = std::round(best_x[j]);
-      continue;
+    max_depth = std::max(max_depth, node.depth);
+    if (has_inc && rel_gap(node.bound, incumbent) <= bo->mip_gap) continue;  // objective cutoff
+    std::vector<double> lb = lb0, ub = ub0;
+    for (const BoundChange& bc : node.bounds) {
+      lb[bc.var] = bc.lower;
+      ub[bc.var] = bc.upper;
     }
-    // Candidate order: most fractional (the reference's FullStrong rule), or
-    // a caller-supplied ranking (the learned surrogate) over the same set.
-    if (bo->candidate_order) {
-      std::vector<double> key(static_cast<std::size_t>(nv), 0.0);
-      bo->candidate_order(nv, node.x.data(), node.lb.data(), node.ub.data(), key.data(), bo->candidate_order_user);
-      std::stable_sort(cand.begin(), cand.end(), [&](int a, int b) { return key[a] > key[b]; });
+    ChildOut lp;
+    if (node.solved) {
+      lp = *node.solved;
+    } else {
+      const auto tl = Clock::now();
+      std::vector<ChildOut> r;
+      const int rc = solve_set({node_spec(lb, ub)}, r);
+      if (rc != SOV_OK) return rc;
+      lp = std::move(r[0]);
+      ++node_lps;
+      lp_seconds += std::chrono::duration<double>(Clock::now() - tl).count();
     }
-    const int k = std::min<int>(static_cast<int>(cand.size()), bo->strong_branch_candidates);
-    cand.resize(static_cast<std::size_t>(k));
-    std::vector<ChildSpec> kids;
-    children_specs(node.lb, node.ub, cand, node.x.data(), nslack, kids);
-    std::vector<ChildOut> outs;
-    const auto ts = Clock::now();
-    const int rc = solve_set(kids, outs);
-    sb_seconds += std::chrono::duration<double>(Clock::now() - ts).count();
-    if (rc != SOV_OK) return rc;
-    const double parent = F.sense * node.bound;  // model sense
-    int best = 0;
-    double best_score = -1.0;
-    for (int q = 0; q < k; ++q) {
-      const double dd = degradation(outs[2 * q], parent), du = degradation(outs[2 * q + 1], parent);
-      const double sc = std::min(dd, du) + 1e-4 * std::max(dd, du);
-      if (sc > best_score) {
-        best_score = sc;
-        best = q;
+    if (lp.status == 2) continue;  // infeasible
+    if (!have_root) {
+      have_root = true;
+      res->root_bound = lp.status == 0 ? lp.objective : 0.0;
+    }
+    const double lp_obj = F.sense * lp.objective;  // min sense
+    if (lp.status == 0 && node.branch_var >= 0 && node.branch_distance > bo->integer_tol) {
+      PseudoCost& p = pseudo[node.branch_var];  // node-LP pseudocost learning (:1159-1170)
+      const double unit = std::abs(lp_obj - node.parent_obj) / node.branch_distance;
+      if (node.branch_up) {
+        p.up_sum += unit;
+        ++p.up_count;
+      } else {
+        p.down_sum += unit;
+        ++p.down_count;
       }
     }
-    for (int side = 0; side < 2; ++side) {
-      const ChildOut& ch = outs[2 * best + side];
-      if (ch.status != 0) continue;  // infeasible (or unresolved) child: no subtree
-      const double b = F.sense * ch.objective;
-      if (b >= incumbent - bo->mip_gap * std::max(1.0, std::abs(incumbent))) continue;
-      std::vector<double> clb(kids[2 * best + side].lb.begin(), kids[2 * best + side].lb.begin() + nv);
-      std::vector<double> cub(kids[2 * best + side].ub.begin(), kids[2 * best + side].ub.begin() + nv);
-      open.push({b, next_id++, std::move(clb), std::move(cub), ch.x});
+    if (lp.status != 0) {  // no certified bound: the subtree is dropped, optimality not certified
+      any_error = true;
+      ++failures;
+      continue;
+    }
+    std::vector<int> frac;
+    for (int j = 0; j < nv; ++j)
+      if (is_int[j] && std::abs(lp.x[j] - std::round(lp.x[j])) > bo->integer_tol) frac.push_back(j);
+    auto snapped = [&]() {  // snap_integer_primal
+      std::vector<double> x = lp.x;
+      for (int j = 0; j < nv; ++j)
+        if (is_int[j] && std::abs(x[j] - std::round(x[j])) <= 10 * bo->integer_tol) x[j] = std::round(x[j]);
+      return x;
+    };
+    if (frac.empty()) {  // integrality before bound pruning (:1317)
+      const int rc = offer(snapped(), lp_obj);
+      if (rc != SOV_OK) return rc;
+      continue;
+    }
+    if (has_inc && rel_gap(lp_obj, incumbent) <= bo->mip_gap) continue;
+
+    const double parent = lp.objective;  // model sense, as the reference's parent_obj
+    PairSolver pairs = [&](const std::vector<int>& vars, std::vector<ChildOut>& outv) {
+      std::vector<ChildSpec> kids;
+      children_specs(lb, ub, vars, lp.x.data(), nslack, kids);
+      return solve_set(kids, outv);
+    };
+    Decision d;
+    int rc = SOV_OK;
+    const auto ts = Clock::now();
+    switch (bo->branch_rule) {
+      case SOV_BRANCH_RELIABILITY:
+        d = pick_reliability(frac, lp.x.data(), parent, bo->integer_tol, bo->reliability_threshold,
+                             bo->max_strong_per_node, pseudo, pairs, &rc, &strong_lps, &strong_used);
+        break;
+      case SOV_BRANCH_FULL_STRONG: {
+        std::vector<double> key;
+        if (bo->candidate_order) {
+          key.assign(static_cast<std::size_t>(nv), 0.0);
+          bo->candidate_order(nv, lp.x.data(), lb.data(), ub.data(), key.data(), bo->candidate_order_user);
+        }
+        long long solved_now = 0;
+        d = pick_full_strong(frac, lp.x.data(), parent, bo->integer_tol, bo->strong_branch_candidates, key, pseudo,
+                             pairs, &rc, &solved_now);
+        strong_lps += solved_now;
+        strong_used += solved_now;
+        break;
+      }
+      case SOV_BRANCH_PSEUDOCOST:
+        d.var = pick_pseudo_cost(frac, lp.x.data(), pseudo);
+        break;
+      default:
+        d.var = pick_most_fractional(frac, lp.x.data());
+    }
+    sb_seconds += std::chrono::duration<double>(Clock::now() - ts).count();
+    if (rc != SOV_OK) return rc;
+    if (d.var < 0) {
+      rc = offer(snapped(), lp_obj);
+      if (rc != SOV_OK) return rc;
+      continue;
+    }
+    const int v = d.var;
+    const double val = lp.x[v], fl = std::floor(val), ce = std::ceil(val);
+    const bool prefer_up = val - fl >= 0.5;  // plunge toward the nearer integer
+    for (const bool up : {prefer_up, !prefer_up}) {
+      const double lower = up ? std::max(lb[v], ce) : lb[v];
+      const double upper = up ? ub[v] : std::min(ub[v], fl);
+      if (lower > upper + 1e-12) continue;
+      OpenNode child;
+      child.bounds = node.bounds;
+      auto same = std::find_if(child.bounds.begin(), child.bounds.end(), [&](const BoundChange& b) { return b.var == v; });
+      if (same != child.bounds.end()) *same = BoundChange{v, lower, upper};
+      else child.bounds.push_back(BoundChange{v, lower, upper});
+      child.id = next_id++;
+      child.depth = node.depth + 1;
+      child.bound = lp_obj;
+      child.branch_var = v;
+      child.branch_up = up;
+      child.branch_distance = up ? ce - val : val - fl;
+      child.parent_obj = lp_obj;
+      if (d.have_children) child.solved = std::make_shared<const ChildOut>(up ? d.up : d.down);
+      if (bo->plunging && up == prefer_up) plunge.push_back(std::move(child));
+      else open.push(std::move(child));
     }
   }
   res->nodes = nodes;
   res->lp_solves = lp_solves;
+  res->node_lp_solves = node_lps;
+  res->strong_lp_solves = strong_lps;
+  res->strong_lp_used = strong_used;
   res->strong_branch_seconds = sb_seconds;
+  res->node_lp_seconds = lp_seconds;
   res->kernel_ms = kernel_ms;
+  res->max_depth = max_depth;
+  res->node_lp_failures = failures;
   res->seconds = std::chrono::duration<double>(Clock::now() - t0).count();
-  if (best_x.empty()) {
-    res->status = limit_hit ? 5 : 2;
+  if (!has_inc) {
+    res->status = any_error ? 6 : limit_hit ? 5 : 2;
     return SOV_OK;
   }
-  res->status = limit_hit ? 1 : 0;
+  res->status = (limit_hit || any_error) ? 1 : 0;
   res->objective = F.sense * incumbent;
   if (x_best)
     for (int j = 0; j < nv; ++j) x_best[j] = best_x[j];

@@ -19,6 +19,11 @@ from .problems import LpModel
 CHILD_BATCHED = 0
 CHILD_PER_LP = 1
 
+# BranchRule (milp/include/sovereign/branch_and_bound.hpp:7-16) and the names
+# SOVEREIGN_BRANCH_RULE takes (milp/src/milp_solver.cpp:13-20).
+BRANCH_RULES = {"most_fractional": 0, "fractional": 0, "strong": 1, "reliability": 1, "pseudocost": 2,
+                "pseudo": 2, "full_strong": 3}
+
 
 class BatchLpSolver:
     """min c'x, A x = b, 0 <= x <= u for many LPs sharing uploaded matrices."""
@@ -113,16 +118,27 @@ def strong_branch(model: LpModel, is_int, x_node, parent_obj, lb=None, ub=None, 
 
 
 class BranchAndBoundSolver:
-    """Best-bound branch-and-bound with strong branching on the GPU.
-    ranker: optional callable(x, lb, ub) -> key per variable (higher first),
-    e.g. the learned surrogate; None keeps the most-fractional order."""
+    """Best-bound branch-and-bound with plunging and strong branching on the
+    GPU (BranchAndBoundOptions, branch_and_bound.hpp:18-59).
+    branch_rule: "strong" (reliability branching, the reference's default),
+    "full_strong", "pseudocost" or "most_fractional".
+    ranker (full_strong): optional callable(x, lb, ub) -> key per variable
+    (higher first), e.g. the learned surrogate; None keeps the
+    closest-to-1/2 order."""
 
     def __init__(self, strong_branch_candidates=64, child_solver=CHILD_BATCHED, max_nodes=100000,
                  mip_gap=1e-6, integer_tol=1e-6, time_limit_seconds=0.0, ranker=None,
-                 options: InteriorPointOptions | None = None):
+                 options: InteriorPointOptions | None = None, branch_rule="strong", reliability_threshold=4,
+                 max_strong_per_node=8, plunging=True):
         self.lib = K.load()
         self.bo = K.BnbOptions()
         self.lib.sov_bnb_default_options(C.byref(self.bo))
+        if branch_rule not in BRANCH_RULES:
+            raise ValueError(f"unknown branch rule {branch_rule!r}")
+        self.bo.branch_rule = BRANCH_RULES[branch_rule]
+        self.bo.reliability_threshold = reliability_threshold
+        self.bo.max_strong_per_node = max_strong_per_node
+        self.bo.plunging = int(bool(plunging))
         self.bo.strong_branch_candidates = strong_branch_candidates
         self.bo.child_solver = child_solver
         self.bo.max_nodes = max_nodes

@@ -157,29 +157,105 @@ def test_batch_uneven_sizes_match_oracle(gpu_api):
     assert bs.last_kernel_ms > 0
 
 
+RULES = ["strong", "full_strong", "pseudocost", "most_fractional"]
+
+
+@pytest.mark.parametrize("rule", RULES)
 @pytest.mark.parametrize("name", ["multi_knapsack_18x3", "multi_knapsack_24x4"])
-def test_bnb_reference_fixtures(gpu_api, name):
+def test_bnb_reference_fixtures(gpu_api, name, rule):
+    """The reference's published optima (102 and 133, EVIDENCE.md:47-50)
+    under every branching rule; "strong" is reliability branching, the
+    reference's default (branch_and_bound.hpp:38)."""
     model = G.load_fixture(name)
     exp = G.milp_expected()[name]["objective"]
-    say(f"bnb {name}")
-    r = MI.BranchAndBoundSolver(strong_branch_candidates=8).solve(model, model.is_int)
-    assert r["status"] == 0
+    say(f"bnb {name} {rule}")
+    r = MI.BranchAndBoundSolver(branch_rule=rule, strong_branch_candidates=4).solve(model, model.is_int)
+    say(f"  nodes {r['nodes']} node LPs {r['node_lp_solves']} strong LPs {r['strong_lp_solves']} "
+        f"(used {r['strong_lp_used']}) depth {r['max_depth']} {r['seconds']:.2f}s")
+    assert r["status"] == 0 and r["node_lp_failures"] == 0
     assert abs(r["objective"] - exp) <= 1e-6 * max(1, abs(exp))
     assert np.all(np.abs(r["x"] - np.round(r["x"])) <= 1e-9)
+    A = model.dense()
+    assert np.all(A @ r["x"] <= model.rhs + 1e-9)
+    assert abs(model.obj @ r["x"] - exp) <= 1e-9 * max(1, abs(exp))
+    assert r["strong_lp_used"] <= r["strong_lp_solves"]
+    if rule in ("pseudocost", "most_fractional"):
+        assert r["strong_lp_solves"] == 0
+    if rule == "strong":
+        assert 0 < r["strong_lp_used"]
 
 
+@pytest.mark.parametrize("rule", ["strong", "full_strong"])
 @pytest.mark.parametrize("seed", [3, 4])
-def test_bnb_identical_incumbent_batched_vs_per_lp(gpu_api, seed):
+def test_bnb_identical_incumbent_batched_vs_per_lp(gpu_api, seed, rule):
     model = P.multi_knapsack(20, 4, seed=seed)
     is_int = np.ones(model.n_vars, dtype=np.int8)
     opt, xopt = scipy_optimum(model, is_int)
-    say(f"bnb mkp 20x4 seed {seed} batched")
-    rb = MI.BranchAndBoundSolver(strong_branch_candidates=4).solve(model, is_int)
-    say(f"bnb mkp 20x4 seed {seed} per-LP (nodes batched: {rb['nodes']})")
-    rp = MI.BranchAndBoundSolver(strong_branch_candidates=4, child_solver=MI.CHILD_PER_LP).solve(model, is_int)
+    say(f"bnb mkp 20x4 seed {seed} {rule} batched")
+    rb = MI.BranchAndBoundSolver(branch_rule=rule, strong_branch_candidates=4).solve(model, is_int)
+    say(f"bnb mkp 20x4 seed {seed} {rule} per-LP (nodes batched: {rb['nodes']})")
+    rp = MI.BranchAndBoundSolver(branch_rule=rule, strong_branch_candidates=4,
+                                 child_solver=MI.CHILD_PER_LP).solve(model, is_int)
     say(f"per-LP nodes {rp['nodes']} {rp['seconds']:.2f}s vs batched {rb['seconds']:.2f}s")
     for r in (rb, rp):
         assert r["status"] == 0
         assert abs(r["objective"] - opt) <= 1e-6 * max(1, abs(opt))
     np.testing.assert_array_equal(rb["x"], rp["x"])
     np.testing.assert_array_equal(rb["x"], np.round(xopt))
+
+
+def knapsack_with_continuous(lb_y):
+    """16 x 3 knapsack plus a continuous y with lower bound lb_y: rows
+    W x <= cap and x0 + x1 - y <= 0.5, maximize profit'x - 3 y."""
+    base = P.multi_knapsack(16, 3, seed=9)
+    n, m = base.n_vars, base.n_rows
+    A = np.zeros((m + 1, n + 1))
+    A[:m, :n] = base.dense()
+    A[m, 0] = A[m, 1] = 1.0
+    A[m, n] = -1.0
+    cols = [list(np.nonzero(A[:, j])[0]) for j in range(n + 1)]
+    ptr, rows, vals = P.csc_from_columns(cols, [list(A[c, j]) for j, c in enumerate(cols)], m + 1)
+    model = P.LpModel(n_vars=n + 1, n_rows=m + 1, col_ptr=ptr, row_idx=rows, values=vals,
+                      obj=np.concatenate([base.obj, [-3.0]]), lb=np.concatenate([np.zeros(n), [lb_y]]),
+                      ub=np.concatenate([np.ones(n), [1e30]]), row_sense=b"L" * (m + 1),
+                      rhs=np.concatenate([base.rhs, [0.5]]), maximize=True)
+    is_int = np.concatenate([np.ones(n), [0]]).astype(np.int8)
+    return model, A, is_int
+
+
+def test_bnb_negative_lower_bound_continuous(gpu_api):
+    """A mixed-integer model whose continuous variable has a finite negative
+    lower bound (shifted into the batched form): batched and per-LP reach
+    scipy's optimum, the incumbent polished over the continuous part
+    (polish_incumbent, branch_and_bound.cpp:522-552)."""
+    model, A, is_int = knapsack_with_continuous(-50.0)
+    n = model.n_vars - 1
+    r = scipy_milp(-model.obj, constraints=LinearConstraint(A, -np.inf, model.rhs), integrality=is_int,
+                   bounds=Bounds(np.concatenate([np.zeros(n), [-50.0]]), np.concatenate([np.ones(n), [np.inf]])))
+    assert r.status == 0
+    opt = -r.fun
+    for solver in (MI.CHILD_BATCHED, MI.CHILD_PER_LP):
+        say(f"bnb negative lower bound child_solver {solver}")
+        res = MI.BranchAndBoundSolver(child_solver=solver).solve(model, is_int)
+        assert res["status"] == 0
+        assert abs(res["objective"] - opt) <= 1e-6 * max(1, abs(opt))
+        x = res["x"]
+        assert np.all(A @ x <= model.rhs + 1e-7)
+        assert abs(model.obj @ x - res["objective"]) <= 1e-6 * max(1, abs(opt))
+
+
+def test_bnb_free_variable_takes_per_lp_path(gpu_api):
+    """A free continuous variable (lb = -1e30) has no shift into the batched
+    child form, so the batched setting sends the model through the per-LP
+    path: both settings must then return the same outcome. The reference's
+    own interior point stops at status Feasible on this model's LP (checked
+    against its compiled sources), so the outcome is a reported failure,
+    never a claimed optimum."""
+    model, _, is_int = knapsack_with_continuous(-1e30)
+    out = []
+    for solver in (MI.CHILD_BATCHED, MI.CHILD_PER_LP):
+        say(f"bnb free variable child_solver {solver}")
+        res = MI.BranchAndBoundSolver(child_solver=solver, max_nodes=50).solve(model, is_int)
+        out.append((res["status"], res["nodes"], res["lp_solves"]))
+    assert out[0] == out[1]
+    assert out[0][0] != 0

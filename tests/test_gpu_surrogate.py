@@ -116,7 +116,7 @@ def test_branch_and_bound_with_surrogate_same_optimum(gpu_api):
                      ("most-fractional top-8", dict(strong_branch_candidates=8)),
                      ("surrogate top-8", dict(strong_branch_candidates=8, ranker=ranker))):
         say(f"bnb mkp 30x6 {name}")
-        runs[name] = MI.BranchAndBoundSolver(**kw).solve(model, is_int)
+        runs[name] = MI.BranchAndBoundSolver(branch_rule="full_strong", **kw).solve(model, is_int)
     ranker.net.close()
     for name, res in runs.items():
         say(f"{name}: nodes {res['nodes']} lp solves {res['lp_solves']} objective {res['objective']}")
