@@ -215,7 +215,7 @@ def branching_leg(lib):
         setups.append(("batched_surrogate_top8", dict(strong_branch_candidates=8, ranker=ranker)))
     for name, kw in setups:
         t0 = time.perf_counter()
-        r = milp.BranchAndBoundSolver(max_nodes=1, **kw).solve(model, is_int)
+        r = milp.BranchAndBoundSolver(max_nodes=1, branch_rule="full_strong", **kw).solve(model, is_int)
         node[name] = {"node_wall_s": time.perf_counter() - t0, "strong_branch_s": r["strong_branch_seconds"],
                       "lp_solves": r["lp_solves"]}
         log(f"root node {name}: {node[name]['node_wall_s']:.3f}s (strong branching {r['strong_branch_seconds']:.3f}s, "
@@ -227,7 +227,57 @@ def branching_leg(lib):
         out["surrogate_top8_contains_full_sb_choice"] = bool(best_b in top8)
         out["wall_per_node_speedup_surrogate_vs_per_lp_full"] = (
             node["per_lp_full_64"]["node_wall_s"] / node["batched_surrogate_top8"]["node_wall_s"])
+    out["bnb_tree"] = tree_leg(model, is_int, ranker)
+    if ranker is not None:
         ranker.net.close()
+    return out
+
+
+def tree_leg(model, is_int, ranker, nodes=40, per_lp_nodes=4):
+    """Wall seconds per branch-and-bound node over a tree (not only the
+    root): the knapsack searched for a fixed node budget with each
+    strong-branching setup, the plain per-LP path on a smaller budget (each
+    of its nodes costs seconds). Then the same setups solve a small knapsack
+    (20 x 4, seed 3) to optimality, where their incumbents must agree."""
+    from sovereign_amd import milp
+
+    setups = [("per_lp_full_64", per_lp_nodes, dict(branch_rule="full_strong", strong_branch_candidates=64,
+                                                    child_solver=milp.CHILD_PER_LP)),
+              ("batched_full_64", nodes, dict(branch_rule="full_strong", strong_branch_candidates=64)),
+              ("batched_reliability", nodes, dict(branch_rule="strong"))]
+    if ranker is not None:
+        setups.append(("batched_surrogate_top8", nodes, dict(branch_rule="full_strong", strong_branch_candidates=8,
+                                                             ranker=ranker)))
+    out = {"model": "multi_knapsack n=500 m=200 seed 26119", "node_budget": nodes, "per_lp_node_budget": per_lp_nodes}
+    for name, budget, kw in setups:
+        r = milp.BranchAndBoundSolver(max_nodes=budget, **kw).solve(model, is_int)
+        out[name] = {"nodes": r["nodes"], "wall_s": r["seconds"], "wall_s_per_node": r["seconds"] / max(1, r["nodes"]),
+                     "lp_solves": r["lp_solves"], "strong_lp_solves": r["strong_lp_solves"],
+                     "node_lp_solves": r["node_lp_solves"], "kernel_ms": r["kernel_ms"], "status": r["status"],
+                     "incumbent": r["objective"] if r["status"] in (0, 1) else None}
+        log(f"tree {name}: {r['nodes']} nodes {r['seconds']:.2f}s ({out[name]['wall_s_per_node'] * 1e3:.1f} ms/node), "
+            f"{r['lp_solves']} LP solves, status {r['status']}")
+    base = out["per_lp_full_64"]["wall_s_per_node"]
+    for name, _, _ in setups[1:]:
+        out[f"per_node_speedup_{name}_vs_per_lp_full"] = base / out[name]["wall_s_per_node"]
+    from sovereign_amd import problems as P
+
+    small = P.multi_knapsack(20, 4, seed=3)
+    sint = np.ones(small.n_vars, dtype=np.int8)
+    objs = {}
+    for name, _, kw in setups:
+        kw = dict(kw)
+        if "ranker" in kw:
+            from sovereign_amd import surrogate as S
+
+            kw["ranker"] = S.load_trained(small)
+        r = milp.BranchAndBoundSolver(**kw).solve(small, sint)
+        objs[name] = {"status": r["status"], "objective": r["objective"], "nodes": r["nodes"], "wall_s": r["seconds"]}
+        if "ranker" in kw:
+            kw["ranker"].net.close()
+    vals = [v["objective"] for v in objs.values()]
+    out["small_instance_to_optimality"] = {"model": "multi_knapsack n=20 m=4 seed 3", "runs": objs,
+                                           "identical_objectives": max(vals) - min(vals) <= 1e-6 * max(1.0, abs(vals[0]))}
     return out
 
 

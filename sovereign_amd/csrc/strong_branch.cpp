@@ -150,54 +150,44 @@ struct ChildSpec {
 // LP relaxation); the LP itself is solved on the node's own bounds.
 class NodePresolve {
  public:
-  explicit NodePresolve(const sov_lp_model& mdl) : m_(mdl.n_rows), n_(mdl.n_vars), sense_(mdl.row_sense, mdl.row_sense + mdl.n_rows), rhs_(mdl.rhs, mdl.rhs + mdl.n_rows) {
+  explicit NodePresolve(const sov_lp_model& mdl)
+      : m_(mdl.n_rows), n_(mdl.n_vars), sense_(mdl.row_sense, mdl.row_sense + mdl.n_rows),
+        rhs_(mdl.rhs, mdl.rhs + mdl.n_rows), cptr_(mdl.col_ptr, mdl.col_ptr + mdl.n_vars + 1),
+        crow_(mdl.row_idx, mdl.row_idx + mdl.col_ptr[mdl.n_vars]), cval_(mdl.values, mdl.values + mdl.col_ptr[mdl.n_vars]) {
     ptr_.assign(static_cast<std::size_t>(m_) + 1, 0);
-    for (int j = 0; j < n_; ++j)
-      for (int p = mdl.col_ptr[j]; p < mdl.col_ptr[j + 1]; ++p) ++ptr_[mdl.row_idx[p] + 1];
+    for (int p = 0; p < cptr_[n_]; ++p) ++ptr_[crow_[p] + 1];
     for (int r = 0; r < m_; ++r) ptr_[r + 1] += ptr_[r];
     col_.resize(static_cast<std::size_t>(ptr_[m_]));
     val_.resize(col_.size());
     std::vector<int> fill(ptr_.begin(), ptr_.end() - 1);
     for (int j = 0; j < n_; ++j)
-      for (int p = mdl.col_ptr[j]; p < mdl.col_ptr[j + 1]; ++p) {
-        const int q = fill[mdl.row_idx[p]]++;
+      for (int p = cptr_[j]; p < cptr_[j + 1]; ++p) {
+        const int q = fill[crow_[p]]++;
         col_[q] = j;
-        val_[q] = mdl.values[p];
+        val_[q] = cval_[p];
       }
   }
 
-  // lb, ub: at least n_vars entries (structural bounds first).
+  // Full test: activity ranges and bound propagation to a fixed point (at
+  // most 8 passes). lb, ub: at least n_vars entries (structural first).
   bool infeasible(std::vector<double> lb, std::vector<double> ub) const {
-    constexpr double kInf = 1e29;
-    auto margin = [](double scale) { return 1e-6 * (1.0 + std::abs(scale)); };
     for (int j = 0; j < n_; ++j)
-      if (lb[j] - ub[j] > margin(std::max(std::abs(lb[j]), std::abs(ub[j])))) return true;
+      if (crossed(lb[j], ub[j])) return true;
     for (int pass = 0; pass < 8; ++pass) {
       bool changed = false;
       for (int r = 0; r < m_; ++r) {
-        const char sn = sense_[r];
-        const bool le = sn == 'L' || sn == 'l' || sn == 'E' || sn == 'e';
-        const bool ge = sn == 'G' || sn == 'g' || sn == 'E' || sn == 'e';
-        double mn = 0.0, mx = 0.0, mag = 0.0;
-        int mn_inf = 0, mx_inf = 0;
-        for (int p = ptr_[r]; p < ptr_[r + 1]; ++p) {
-          const double a = val_[p];
-          const int j = col_[p];
-          const double lo = a > 0 ? lb[j] : ub[j], hi = a > 0 ? ub[j] : lb[j];
-          if (std::abs(lo) >= kInf) ++mn_inf; else mn += a * lo;
-          if (std::abs(hi) >= kInf) ++mx_inf; else mx += a * hi;
-          mag += (std::abs(lo) >= kInf ? 0.0 : std::abs(a * lo)) + (std::abs(hi) >= kInf ? 0.0 : std::abs(a * hi));
-        }
-        const double mg = margin(mag);
-        if (le && mn_inf == 0 && mn > rhs_[r] + mg) return true;
-        if (ge && mx_inf == 0 && mx < rhs_[r] - mg) return true;
+        const bool le = is_le(sense_[r]), ge = is_ge(sense_[r]);
+        Activity act;
+        for (int p = ptr_[r]; p < ptr_[r + 1]; ++p) act.add(val_[p], lb[col_[p]], ub[col_[p]], 1.0);
+        if (violates(act, r)) return true;
+        const double mn = act.mn, mx = act.mx;
         // a_j x_j <= rhs - (min activity of the rest), >= rhs - (max of the rest)
         for (int p = ptr_[r]; p < ptr_[r + 1]; ++p) {
           const double a = val_[p];
           const int j = col_[p];
           const double lo = a > 0 ? lb[j] : ub[j], hi = a > 0 ? ub[j] : lb[j];
           const bool lo_inf = std::abs(lo) >= kInf, hi_inf = std::abs(hi) >= kInf;
-          if (le && mn_inf - (lo_inf ? 1 : 0) == 0) {
+          if (le && act.mn_inf - (lo_inf ? 1 : 0) == 0) {
             const double cand = (rhs_[r] - (mn - (lo_inf ? 0.0 : a * lo))) / a;
             if (std::abs(cand) <= 1e9) {
               double& bnd = a > 0 ? ub[j] : lb[j];
@@ -207,7 +197,7 @@ class NodePresolve {
               }
             }
           }
-          if (ge && mx_inf - (hi_inf ? 1 : 0) == 0) {
+          if (ge && act.mx_inf - (hi_inf ? 1 : 0) == 0) {
             const double cand = (rhs_[r] - (mx - (hi_inf ? 0.0 : a * hi))) / a;
             if (std::abs(cand) <= 1e9) {
               double& bnd = a > 0 ? lb[j] : ub[j];
@@ -217,7 +207,7 @@ class NodePresolve {
               }
             }
           }
-          if (lb[j] - ub[j] > margin(std::max(std::abs(lb[j]), std::abs(ub[j])))) return true;
+          if (crossed(lb[j], ub[j])) return true;
         }
       }
       if (!changed) break;
@@ -225,17 +215,81 @@ class NodePresolve {
     return false;
   }
 
-  // Marks the children whose LP this test proves infeasible.
+  // Quick test of a set of children of one node (their bounds differ in a
+  // few variables): each row's activity range under the first child's
+  // bounds, updated column by column for the others. Marks the children it
+  // proves infeasible.
   template <typename Spec>
   void mark(std::vector<Spec>& kids) const {
-    for (Spec& k : kids)
-      if (!k.empty && infeasible(k.lb, k.ub)) k.empty = true;
+    if (kids.empty() || m_ == 0) return;
+    const Spec& k0 = kids[0];
+    std::vector<Activity> base(static_cast<std::size_t>(m_));
+    for (int j = 0; j < n_; ++j)
+      for (int p = cptr_[j]; p < cptr_[j + 1]; ++p) base[crow_[p]].add(cval_[p], k0.lb[j], k0.ub[j], 1.0);
+    std::vector<Activity> act;
+    std::vector<int> rows;
+    for (Spec& k : kids) {
+      if (k.empty) continue;
+      bool bad = false;
+      rows.clear();
+      act = base;
+      for (int j = 0; j < n_ && !bad; ++j) {
+        if (k.lb[j] == k0.lb[j] && k.ub[j] == k0.ub[j]) continue;
+        bad = crossed(k.lb[j], k.ub[j]);
+        for (int p = cptr_[j]; p < cptr_[j + 1]; ++p) {
+          act[crow_[p]].add(cval_[p], k0.lb[j], k0.ub[j], -1.0);
+          act[crow_[p]].add(cval_[p], k.lb[j], k.ub[j], 1.0);
+          rows.push_back(crow_[p]);
+        }
+      }
+      const bool first = &k == &kids[0];
+      if (first)
+        for (int r = 0; r < m_ && !bad; ++r) bad = violates(act[r], r);
+      else
+        for (int r : rows)
+          if (!bad) bad = violates(act[r], r);
+      if (bad) k.empty = true;
+    }
   }
 
  private:
+  static constexpr double kInf = 1e29;
+  static double margin(double scale) { return 1e-6 * (1.0 + std::abs(scale)); }
+  static bool crossed(double lo, double hi) { return lo - hi > margin(std::max(std::abs(lo), std::abs(hi))); }
+  static bool is_le(char c) { return c == 'L' || c == 'l' || c == 'E' || c == 'e'; }
+  static bool is_ge(char c) { return c == 'G' || c == 'g' || c == 'E' || c == 'e'; }
+
+  // A row's activity range: finite parts, counts of infinite terms, and the
+  // magnitude that scales its margin.
+  struct Activity {
+    double mn = 0.0, mx = 0.0, mag = 0.0;
+    int mn_inf = 0, mx_inf = 0;
+    void add(double a, double lb, double ub, double w) {
+      const double lo = a > 0 ? lb : ub, hi = a > 0 ? ub : lb;
+      if (std::abs(lo) >= kInf) mn_inf += static_cast<int>(w);
+      else {
+        mn += w * a * lo;
+        mag += w * std::abs(a * lo);
+      }
+      if (std::abs(hi) >= kInf) mx_inf += static_cast<int>(w);
+      else {
+        mx += w * a * hi;
+        mag += w * std::abs(a * hi);
+      }
+    }
+  };
+  bool violates(const Activity& act, int r) const {
+    const double mg = margin(act.mag);
+    if (is_le(sense_[r]) && act.mn_inf == 0 && act.mn > rhs_[r] + mg) return true;
+    if (is_ge(sense_[r]) && act.mx_inf == 0 && act.mx < rhs_[r] - mg) return true;
+    return false;
+  }
+
   int m_, n_;
   std::vector<char> sense_;
   std::vector<double> rhs_;
+  std::vector<int> cptr_, crow_;
+  std::vector<double> cval_;
   std::vector<int> ptr_, col_;
   std::vector<double> val_;
 };
@@ -346,6 +400,15 @@ int finish_unconverged(const sov_lp_model& mdl, const std::vector<ChildSpec>& ki
   const int rc = solve_children_one_by_one(mdl, redo, F, opt, again, lp_solves);
   if (rc != SOV_OK) return rc;
   for (std::size_t q = 0; q < where.size(); ++q) out[where[q]] = std::move(again[q]);
+  return SOV_OK;
+}
+
+// An LP the interior point left unconverged may be infeasible, which the
+// interior point cannot certify: the full presolve test settles the ones
+// that bound propagation proves empty.
+int prove_infeasible(const NodePresolve& pre, const std::vector<ChildSpec>& kids, std::vector<ChildOut>& out) {
+  for (std::size_t k = 0; k < out.size(); ++k)
+    if (out[k].status != 0 && out[k].status != 2 && pre.infeasible(kids[k].lb, kids[k].ub)) out[k].status = 2;
   return SOV_OK;
 }
 
@@ -650,7 +713,8 @@ int sov_strong_branch(const sov_lp_model* node, const double* node_lb, const dou
   std::vector<double> lb(node_lb, node_lb + node->n_vars), ub(node_ub, node_ub + node->n_vars);
   std::vector<ChildSpec> kids;
   children_specs(lb, ub, cand, x_node, F.n - F.nv, kids);
-  NodePresolve(*node).mark(kids);
+  const NodePresolve presolve(*node);
+  presolve.mark(kids);
   std::vector<ChildOut> res;
   int rc;
   if (child_solver == SOV_CHILD_BATCHED && batched_form_ok(node_lb, node->n_vars)) {
@@ -660,9 +724,11 @@ int sov_strong_branch(const sov_lp_model* node, const double* node_lb, const dou
     rc = sov_batch_add_matrix(batch, F.m, F.n, F.A.data(), &mid);
     if (rc == SOV_OK) rc = solve_children(batch, mid, F, kids, *opt, res, kernel_ms, nullptr);
     sov_batch_free(batch);
+    if (rc == SOV_OK) rc = prove_infeasible(presolve, kids, res);
     if (rc == SOV_OK) rc = finish_unconverged(*node, kids, F, *opt, res, nullptr);
   } else {
     rc = solve_children_one_by_one(*node, kids, F, *opt, res, nullptr);
+    if (rc == SOV_OK) rc = prove_infeasible(presolve, kids, res);
   }
   if (rc != SOV_OK) return rc;
   double best_score = -1.0;
@@ -723,10 +789,12 @@ int sov_bnb_solve(const sov_lp_model* milp, const int8_t* is_int, const sov_bnb_
   auto solve_set = [&](std::vector<ChildSpec> kids, std::vector<ChildOut>& outv) {
     presolve.mark(kids);
     if (batched) {
-      const int rc = solve_children(batch.get(), mat_id, F, kids, *opt, outv, &kernel_ms, &lp_solves);
+      int rc = solve_children(batch.get(), mat_id, F, kids, *opt, outv, &kernel_ms, &lp_solves);
+      if (rc == SOV_OK) rc = prove_infeasible(presolve, kids, outv);
       return rc != SOV_OK ? rc : finish_unconverged(*milp, kids, F, *opt, outv, &lp_solves);
     }
-    return solve_children_one_by_one(*milp, kids, F, *opt, outv, &lp_solves);
+    const int rc = solve_children_one_by_one(*milp, kids, F, *opt, outv, &lp_solves);
+    return rc != SOV_OK ? rc : prove_infeasible(presolve, kids, outv);
   };
   auto node_spec = [&](const std::vector<double>& lb, const std::vector<double>& ub) {
     ChildSpec s;
