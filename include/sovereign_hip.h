@@ -98,6 +98,20 @@ int sov_gemm_shapes(int32_t* shapes4, double* times_ms12);
  * the largest remaining pivot magnitude is below 1e-14. */
 int sov_dense_lu(int64_t n, double* a_host, int32_t* piv_host);
 
+/* Allocation-free form of sov_dense_lu, on device memory. No counterpart in
+ * the reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc). a_dev (n x n, column-major) is factored in place, piv_dev (n
+ * int32) receives DenseLU's permutation; results equal sov_dense_lu's on the
+ * same matrix. The pivot swap list and the singular flag live in a fixed
+ * device array of the library (one per GPU, held until the process ends),
+ * so n is at most 32768 (larger n: SOV_ERR_ARG) and the call allocates
+ * nothing. Calls that overlap on one GPU (other host threads, other
+ * streams) share that array and race; calls are serialised on the library
+ * stream, which this call synchronises before it returns the status
+ * (0 ok, 1 singular). */
+int sov_dense_lu_dev_noalloc(int64_t n, double* a_dev, int32_t* piv_dev);
+
 /* ------------------------------------------------------------------ SpMV --
  * Replaces cuda::spmv_csc (cuda_driver.hpp:36-40) and its content-compared
  * per-thread MatrixCache (cuda_driver.cpp:509-582) by an explicit handle:

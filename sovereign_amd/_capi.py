@@ -178,6 +178,7 @@ SIGNATURES = {
     "sov_kernel_times": (C.c_int, [_f64p]),
     "sov_gemm_shapes": (C.c_int, [_i32p, _f64p]),
     "sov_dense_lu": (C.c_int, [C.c_int64, _f64p, _i32p]),
+    "sov_dense_lu_dev_noalloc": (C.c_int, [C.c_int64, C.c_void_p, C.c_void_p]),
     "sov_mat_upload_csc": (C.c_int, [C.c_int32, C.c_int32, _i32p, _i32p, _f64p, C.POINTER(_vp)]),
     "sov_mat_free": (C.c_int, [_vp]),
     "sov_spmv": (C.c_int, [_vp, C.c_int32, _f64p, _f64p]),

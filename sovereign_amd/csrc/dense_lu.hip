@@ -16,6 +16,8 @@
 #include "mfma_gemm.hpp"
 
 #include <algorithm>
+#include <mutex>
+#include <string>
 #include <vector>
 
 namespace {
@@ -133,9 +135,58 @@ __global__ __launch_bounds__(256) void lu_trsm_kernel(double* __restrict__ a, lo
     if (i < nb) col[i] = v[i];
 }
 
+// Clears the singular flag: a kernel rather than hipMemsetAsync, so that the
+// sequence stays free of memset nodes.
+__global__ void lu_clear_kernel(int* __restrict__ status) { *status = 0; }
+
+// LAPACK-style swap list -> DenseLU's permutation (row i of P*A is row
+// piv[i] of A; cuda_driver.cpp:496-500): a sequential walk, one thread.
+__global__ void lu_perm_kernel(const int32_t* __restrict__ rows, int32_t* __restrict__ piv, long long n) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  for (long long i = 0; i < n; ++i) piv[i] = static_cast<int32_t>(i);
+  for (long long k = 0; k < n; ++k) {
+    const int32_t r = rows[k];
+    const int32_t t = piv[k];
+    piv[k] = piv[r];
+    piv[r] = t;
+  }
+}
+
+// Scratch of the allocation-free entry point: the swap list (kNoAllocMaxN
+// entries) and, after it, the singular flag. Each GPU holds its own copy;
+// host code reaches it only through the address hipGetSymbolAddress returns.
+constexpr int64_t kNoAllocMaxN = 32768;
+__device__ __align__(256) int32_t scratch[kNoAllocMaxN + 64];
+
 }  // namespace
 
 namespace sov {
+
+// The factorization's launches on device memory: a (n x n, column-major) in
+// place, the swap list in ipiv (n), the singular flag in *status (cleared
+// here). Nothing is allocated.
+int dense_lu_launch(int64_t n, double* a, int32_t* ipiv, int* status, hipStream_t s) {
+  lu_clear_kernel<<<1, 1, 0, s>>>(status);
+  SOV_LAUNCH_CHECK();
+  for (int64_t k0 = 0; k0 < n; k0 += kNb) {
+    const int nb = static_cast<int>(std::min<int64_t>(kNb, n - k0));
+    lu_panel_kernel<<<1, 1024, 0, s>>>(a, n, static_cast<int>(k0), nb, ipiv, status);
+    SOV_LAUNCH_CHECK();
+    lu_swap_kernel<<<blocks_for(n, 256), 256, 0, s>>>(a, n, static_cast<int>(k0), nb, ipiv);
+    SOV_LAUNCH_CHECK();
+    const int64_t rest = n - k0 - nb;
+    if (rest > 0) {
+      lu_trsm_kernel<<<blocks_for(rest, 256), 256, 0, s>>>(a, n, static_cast<int>(k0), nb);
+      SOV_LAUNCH_CHECK();
+      const double* l21 = a + (k0 + nb) + k0 * n;
+      const double* u12 = a + k0 + (k0 + nb) * n;
+      double* a22 = a + (k0 + nb) + (k0 + nb) * n;
+      SOV_HIP_TRY((gemm_update<double, true, false>(static_cast<int>(rest), static_cast<int>(rest), nb, l21, n, u12,
+                                                    n, a22, n, -1.0, s)));
+    }
+  }
+  return SOV_OK;
+}
 
 int dense_lu_run(int64_t n, double* a_host, int32_t* piv_host) {
   if (n <= 0 || !a_host || !piv_host) return SOV_ERR_ARG;
@@ -157,25 +208,8 @@ int dense_lu_run(int64_t n, double* a_host, int32_t* piv_host) {
   SOV_HIP_TRY(hipMalloc(&a, bytes));
   SOV_HIP_TRY(hipMalloc(&ipiv, static_cast<std::size_t>(n) * sizeof(int32_t)));
   SOV_HIP_TRY(hipMalloc(&status, sizeof(int)));
-  SOV_HIP_TRY(hipMemsetAsync(status, 0, sizeof(int), s));
   SOV_HIP_TRY(hipMemcpyAsync(a, a_host, bytes, hipMemcpyHostToDevice, s));
-  for (int64_t k0 = 0; k0 < n; k0 += kNb) {
-    const int nb = static_cast<int>(std::min<int64_t>(kNb, n - k0));
-    lu_panel_kernel<<<1, 1024, 0, s>>>(a, n, static_cast<int>(k0), nb, ipiv, status);
-    SOV_LAUNCH_CHECK();
-    lu_swap_kernel<<<blocks_for(n, 256), 256, 0, s>>>(a, n, static_cast<int>(k0), nb, ipiv);
-    SOV_LAUNCH_CHECK();
-    const int64_t rest = n - k0 - nb;
-    if (rest > 0) {
-      lu_trsm_kernel<<<blocks_for(rest, 256), 256, 0, s>>>(a, n, static_cast<int>(k0), nb);
-      SOV_LAUNCH_CHECK();
-      const double* l21 = a + (k0 + nb) + k0 * n;
-      const double* u12 = a + k0 + (k0 + nb) * n;
-      double* a22 = a + (k0 + nb) + (k0 + nb) * n;
-      SOV_HIP_TRY((gemm_update<double, true, false>(static_cast<int>(rest), static_cast<int>(rest), nb, l21, n, u12,
-                                                    n, a22, n, -1.0, s)));
-    }
-  }
+  if (const int rc = dense_lu_launch(n, a, ipiv, status, s); rc != SOV_OK) return rc;
   int st = 0;
   SOV_HIP_TRY(hipMemcpyAsync(&st, status, sizeof(int), hipMemcpyDeviceToHost, s));
   SOV_HIP_TRY(hipStreamSynchronize(s));
@@ -202,4 +236,60 @@ extern "C" int sov_dense_lu(int64_t n, double* a_host, int32_t* piv_host) {
     sov::count_factorization();
   }
   return rc;
+}
+
+namespace {
+
+// Per-device address of `scratch`, looked up once per device with that
+// device current.
+std::mutex g_scratch_mu;
+int32_t* g_scratch_dev[64] = {};
+
+int scratch_for_current_device(int32_t** out) {
+  int dev = 0;
+  SOV_HIP_TRY(hipGetDevice(&dev));
+  if (dev < 0 || dev >= 64) {
+    sov::set_error("sov_dense_lu_dev_noalloc: device index out of range");
+    return SOV_ERR_DEVICE;
+  }
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_dev[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      sov::set_error(std::string("sov_dense_lu_dev_noalloc: hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+      return SOV_ERR_DEVICE;
+    }
+    g_scratch_dev[dev] = static_cast<int32_t*>(p);
+  }
+  *out = g_scratch_dev[dev];
+  return SOV_OK;
+}
+
+}  // namespace
+
+extern "C" int sov_dense_lu_dev_noalloc(int64_t n, double* a_dev, int32_t* piv_dev) {
+  if (n <= 0 || !a_dev || !piv_dev) return SOV_ERR_ARG;
+  if (n > kNoAllocMaxN) {
+    sov::set_error("sov_dense_lu_dev_noalloc: n exceeds the scratch limit of 32768");
+    return SOV_ERR_ARG;
+  }
+  if (!sov::device_ready()) {
+    sov::set_error("no usable GPU");
+    return SOV_ERR_NODEV;
+  }
+  int32_t* scr = nullptr;
+  if (const int rc = scratch_for_current_device(&scr); rc != SOV_OK) return rc;
+  int32_t* ipiv = scr;
+  int* status = reinterpret_cast<int*>(scr + kNoAllocMaxN);
+  hipStream_t s = sov::stream();
+  if (const int rc = sov::dense_lu_launch(n, a_dev, ipiv, status, s); rc != SOV_OK) return rc;
+  lu_perm_kernel<<<1, 64, 0, s>>>(ipiv, piv_dev, n);
+  SOV_LAUNCH_CHECK();
+  int st = 0;
+  SOV_HIP_TRY(hipMemcpyAsync(&st, status, sizeof(int), hipMemcpyDeviceToHost, s));
+  SOV_HIP_TRY(hipStreamSynchronize(s));
+  sov::count_operation();
+  sov::count_factorization();
+  return st != 0 ? SOV_SINGULAR : SOV_OK;
 }
