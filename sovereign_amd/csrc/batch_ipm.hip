@@ -10,12 +10,14 @@
 //                                            u_j == 0: x_j fixed at 0)
 //
 // is solved by a bounded-variable Mehrotra predictor-corrector method run
-// entirely by one 1024-thread workgroup; workgroups take LPs from a queue, so
+// entirely by one 512-thread workgroup; workgroups take LPs from a queue, so
 // LPs of uneven size share one launch. Per iteration:
 //   M = A D A^T          FP32 MFMA (v_mfma_f32_16x16x4f32), FP32 accumulation,
 //                        written to LDS as a packed lower triangle (m <= 272);
-//   M = L L^T            FP32, in LDS, with the LDL^T pivot rule of the
-//                        reference (drop a pivot <= 1e-13 |diagonal|);
+//   M = L L^T            FP32, in LDS, blocked 16 columns a step (diagonal
+//                        block in registers, panel and trailing update on
+//                        MFMA), with the LDL^T pivot rule of the reference
+//                        (drop a pivot <= 1e-13 |diagonal|);
 //   M dy = rhs           FP64 preconditioned CG with the FP32 factor as the
 //                        preconditioner and the FP64 product A (D (A^T p)).
 // Residuals, step lengths and the termination test are FP64.
@@ -23,6 +25,7 @@
 #include "internal.hpp"
 
 #include <algorithm>
+#include <cstdint>
 #include <cstdio>
 #include <cstdlib>
 #include <cmath>
@@ -33,11 +36,11 @@ typedef double dbl4_t __attribute__((ext_vector_type(4)));
 
 namespace {
 
-// 16 wavefronts. At this size the kernel spills about 850 B per thread (128
-// registers a wave); 512 threads remove the spills but take longer per LP
-// (85 against 77 ms for the 128 knapsack children): the LDS factorization
-// and inversion steps have half the threads.
-constexpr int kThreads = 1024;
+// 8 wavefronts: 256 registers a lane, 224 B of scratch a lane. (At 1024
+// threads, 128 registers a lane, the kernel spilled over 1 KB a lane; that
+// size was preferred while the LDS factorization went a column at a time
+// and wanted every thread.)
+constexpr int kThreads = 512;
 static_assert(kThreads % 256 == 0 && kThreads <= 1024, "gemv_n slices the columns in 256-row groups");
 constexpr int kWaves = kThreads / 64;
 constexpr int kMaxRows = 272;
@@ -46,9 +49,13 @@ constexpr int kTraceRows = 200, kTraceCols = 16;  // SOVEREIGN_BATCH_TRACE recor
 
 struct BatchMatrix {
   int m, n;
-  const double* a64;  // column-major m x n
-  const float* a32;
+  const double* a64;   // column-major m x n
+  const float* a32;    // row-major m x ldr, zero beyond column n (FP32 operands of the assembly)
+  int ldr;             // n rounded up to a multiple of 32
+  const double* a64r;  // row-major m x ldr (FP64, for A^T v)
 };
+
+__host__ __device__ constexpr int row_ld(int n) { return (n + 31) / 32 * 32; }
 
 struct BatchDesc {
   int mat;
@@ -88,12 +95,16 @@ __device__ __forceinline__ double bred(double v, double* red) {
 struct Lp {
   int m, n;
   const double* A;
-  const float* Af;
+  const float* Af;   // row-major, leading dimension ldr
+  const double* Ar;  // row-major FP64, leading dimension ldr
+  int ldr;
   const double *b, *c, *u;
   // workspace vectors (global, this LP only)
   double *x, *w, *z, *v, *rc, *ru, *D, *dx, *dw, *dz, *dv, *dxa, *dwa, *dza, *dva, *t;
   double *y, *rb, *dy, *r, *p, *q, *zz, *rhs, *sc;  // sc: Jacobi scale of A D A^T
   double* M64;  // packed lower FP64 normal matrix (global), for LPs whose FP32 solves stall
+  float* xs;    // inverses of the FP32 factor's 16 x 16 diagonal blocks (global, 256 floats each)
+  float* sd;    // sqrt(D) in FP32, ldr entries (zero beyond n)
 };
 
 // Storage of the packed normal matrix and its factor: FP32 in LDS, or FP64
@@ -108,7 +119,6 @@ struct Prec<float> {
   }
   typedef flt4_t acc_t;
   __device__ static __forceinline__ int row(int lk, int g) { return 4 * lk + g; }  // output row of register g
-  __device__ static __forceinline__ float a(const Lp& L, long long idx) { return L.Af[idx]; }
 };
 template <>
 struct Prec<double> {
@@ -121,45 +131,57 @@ struct Prec<double> {
   __device__ static __forceinline__ double a(const Lp& L, long long idx) { return L.A[idx]; }
 };
 
-// out_j = (A^T vec)_j for j < n; vec (m) is staged in LDS first.
+// out_j = (A^T vec)_j for j < n; vec (m) is staged in LDS first. One thread
+// per column over the row-major FP64 copy (consecutive threads read
+// consecutive columns of a row), four partial sums over rows i mod 4.
 __device__ void gemv_t(const Lp& L, const double* vec, double* out, double* stage) {
-  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int tid = threadIdx.x;
   __syncthreads();
   for (int i = tid; i < L.m; i += kThreads) stage[i] = vec[i];
   __syncthreads();
-  for (int j0 = wv * 4; j0 < L.n; j0 += kWaves * 4) {
-    double s[4] = {0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      const int j = j0 + q;
-      if (j < L.n) {
-        const double* col = L.A + static_cast<long long>(j) * L.m;
-        for (int i = lane; i < L.m; i += 64) s[q] = fma(col[i], stage[i], s[q]);
-      }
+  const int m = L.m, ldr = L.ldr;
+  for (int j = tid; j < L.n; j += kThreads) {
+    const double* col = L.Ar + j;
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+    int i = 0;
+    for (; i + 4 <= m; i += 4) {
+      s0 = fma(col[static_cast<long long>(i) * ldr], stage[i], s0);
+      s1 = fma(col[static_cast<long long>(i + 1) * ldr], stage[i + 1], s1);
+      s2 = fma(col[static_cast<long long>(i + 2) * ldr], stage[i + 2], s2);
+      s3 = fma(col[static_cast<long long>(i + 3) * ldr], stage[i + 3], s3);
     }
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      const double tot = wave_sum(s[q]);
-      if (lane == 0 && j0 + q < L.n) out[j0 + q] = tot;
-    }
+    for (; i < m; ++i) s0 = fma(col[static_cast<long long>(i) * ldr], stage[i], s0);
+    out[j] = (s0 + s1) + (s2 + s3);
   }
   __syncthreads();
 }
 
 // out_i = base_i + (A vec)_i for i < m (base may be null). vec (n) global.
+// Threads take rows (consecutive threads read consecutive rows of a column),
+// kThreads / 256 slices of the columns, four partial sums per thread.
 __device__ void gemv_n(const Lp& L, const double* vec, const double* base, double* out, double* stage) {
   constexpr int kSlices = kThreads / 256;  // slices of the columns, 256 rows each
   const int tid = threadIdx.x;
   const int row = tid & 255, h = tid >> 8;
-  for (int r0 = 0; r0 < L.m; r0 += 256) {
+  const int m = L.m, n = L.n;
+  for (int r0 = 0; r0 < m; r0 += 256) {
     const int i = r0 + row;
-    double acc = 0.0;
-    if (i < L.m)
-      for (int j = h; j < L.n; j += kSlices) acc = fma(L.A[i + static_cast<long long>(j) * L.m], vec[j], acc);
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+    if (i < m) {
+      const double* ai = L.A + i;
+      int j = h;
+      for (; j + 3 * kSlices < n; j += 4 * kSlices) {
+        a0 = fma(ai[static_cast<long long>(j) * m], vec[j], a0);
+        a1 = fma(ai[static_cast<long long>(j + kSlices) * m], vec[j + kSlices], a1);
+        a2 = fma(ai[static_cast<long long>(j + 2 * kSlices) * m], vec[j + 2 * kSlices], a2);
+        a3 = fma(ai[static_cast<long long>(j + 3 * kSlices) * m], vec[j + 3 * kSlices], a3);
+      }
+      for (; j < n; j += kSlices) a0 = fma(ai[static_cast<long long>(j) * m], vec[j], a0);
+    }
     __syncthreads();
-    stage[h * 256 + row] = acc;
+    stage[h * 256 + row] = (a0 + a1) + (a2 + a3);
     __syncthreads();
-    if (h == 0 && i < L.m) {
+    if (h == 0 && i < m) {
       double s = stage[row];
 #pragma unroll
       for (int q = 1; q < kSlices; ++q) s += stage[q * 256 + row];
@@ -169,7 +191,75 @@ __device__ void gemv_n(const Lp& L, const double* vec, const double* base, doubl
   __syncthreads();
 }
 
-// Jacobi-scaled normal matrix in LDS (packed lower, FP32):
+// Jacobi-scaled FP32 normal matrix in LDS (packed lower):
+//   Mt = S A D A^T S,  S = diag(sc),  sc_i = (A D A^T)_ii^(-1/2),
+// on MFMA from the operands A(i, k) sqrt(D_k) (the row-major FP32 copy of A
+// and sqrt(D) in FP32, both read 16 bytes a lane: lane group g takes
+// k = k0 + 4 g .. k0 + 4 g + 3, the same k for both operands), S applied to
+// each tile's sums. sc comes from a row-wise FP32 sum of the same operands,
+// one wavefront a row. Two 16-column chunks are loaded ahead of their MFMAs.
+__device__ void assemble32(const Lp& L, float* M, double reg) {
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int lrow = lane & 15, lk = lane >> 4;
+  const int m = L.m, n = L.n, ldr = L.ldr;
+  for (int k = tid; k < ldr; k += kThreads) L.sd[k] = k < n ? static_cast<float>(sqrt(fmax(L.D[k], 0.0))) : 0.f;
+  __syncthreads();
+  for (int i = wv; i < m; i += kWaves) {
+    const float* ar = L.Af + static_cast<long long>(i) * ldr;
+    float acc = 0.f;
+    for (int k = 4 * lane; k < ldr; k += 256) {
+      const flt4_t a = *reinterpret_cast<const flt4_t*>(ar + k);
+      const flt4_t d = *reinterpret_cast<const flt4_t*>(L.sd + k);
+#pragma unroll
+      for (int u = 0; u < 4; ++u) acc += (a[u] * d[u]) * (a[u] * d[u]);
+    }
+    const double dii = static_cast<double>(wave_sum(static_cast<double>(acc)));
+    if (lane == 0) L.sc[i] = dii + reg > 0.0 ? 1.0 / sqrt(dii + reg) : 1.0;
+  }
+  __syncthreads();
+  const int nt = (m + 15) / 16;
+  const int tiles = nt * (nt + 1) / 2;
+  const flt4_t zero = {0.f, 0.f, 0.f, 0.f};
+  for (int t = wv; t < tiles; t += kWaves) {
+    int ti = static_cast<int>((sqrt(8.0 * t + 1.0) - 1.0) * 0.5);
+    while ((ti + 1) * (ti + 2) / 2 <= t) ++ti;
+    while (ti * (ti + 1) / 2 > t) --ti;
+    const int tj = t - ti * (ti + 1) / 2;
+    const int r0 = ti * 16, c0 = tj * 16;
+    const int ra = r0 + lrow, cb = c0 + lrow;
+    const float* pa = L.Af + static_cast<long long>(ra < m ? ra : 0) * ldr + 4 * lk;
+    const float* pb = L.Af + static_cast<long long>(cb < m ? cb : 0) * ldr + 4 * lk;
+    const float* pd = L.sd + 4 * lk;
+    const bool va = ra < m, vb = cb < m;
+    flt4_t acc = zero;
+    for (int k0 = 0; k0 < ldr; k0 += 32) {  // ldr is a multiple of 32
+      const flt4_t a0 = va ? *reinterpret_cast<const flt4_t*>(pa + k0) : zero;
+      const flt4_t b0 = vb ? *reinterpret_cast<const flt4_t*>(pb + k0) : zero;
+      const flt4_t d0 = *reinterpret_cast<const flt4_t*>(pd + k0);
+      const flt4_t a1 = va ? *reinterpret_cast<const flt4_t*>(pa + k0 + 16) : zero;
+      const flt4_t b1 = vb ? *reinterpret_cast<const flt4_t*>(pb + k0 + 16) : zero;
+      const flt4_t d1 = *reinterpret_cast<const flt4_t*>(pd + k0 + 16);
+#pragma unroll
+      for (int u = 0; u < 4; ++u) acc = Prec<float>::mfma(a0[u] * d0[u], b0[u] * d0[u], acc);
+#pragma unroll
+      for (int u = 0; u < 4; ++u) acc = Prec<float>::mfma(a1[u] * d1[u], b1[u] * d1[u], acc);
+    }
+    // lane holds column c0 + lrow, rows r0 + Prec<float>::row(lk, g)
+    const double sj = cb < m ? L.sc[cb] : 0.0;
+#pragma unroll
+    for (int g = 0; g < 4; ++g) {
+      const int i = r0 + Prec<float>::row(lk, g), j = cb;
+      if (i < m && j < m && i >= j) {
+        const double si = L.sc[i];
+        M[packed_index(i, j, m)] = static_cast<float>(si * sj * (static_cast<double>(acc[g]) + (i == j ? reg : 0.0)));
+      }
+    }
+  }
+  __syncthreads();
+}
+
+// Jacobi-scaled normal matrix (packed lower; the FP64 copy in global memory
+// for an LP whose FP32-preconditioned solves stalled):
 //   Mt = S A D A^T S,  S = diag(sc),  sc_i = (A D A^T)_ii^(-1/2),
 // assembled on MFMA from the operands A(i, k) sqrt(D_k) sc_i, so Mt has a
 // unit diagonal whatever the spread of D; the FP32 factorization of Mt is
@@ -330,6 +420,230 @@ __device__ void precondition(const Lp& L, const T* M, double* vec, double* tmp) 
   __syncthreads();
 }
 
+// Orders one wavefront's LDS accesses (no workgroup barrier).
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+}
+
+constexpr int kBlk = 16;  // block width of the blocked LDS factorization and inversion
+// the inversion gives each block row below a block column a wavefront slot
+constexpr int kRowsPerWave = ((kMaxRows + kBlk - 1) / kBlk - 1 + kWaves - 1) / kWaves;
+
+// Element (i, k) of the packed lower triangle as an MFMA operand: zero above
+// the diagonal and outside the matrix.
+__device__ __forceinline__ float packed_at(const float* M, int i, int k, int m) {
+  return (i < m && k <= i) ? M[packed_index(i, k, m)] : 0.f;
+}
+
+// Factor L and inverse X = L^-1 of the kb x kb (kb <= 16) diagonal block at
+// (k0, k0) of the packed matrix, by one wavefront in registers. Lane l holds
+// column c = l & 15, rows r_g = (l >> 4) + 4 g, g < 4, of the block (f) and of
+// X (x). Step j takes the pivot (the pivot rule of cholesky() above), scales
+// column j of L and row j of X, then applies f -= L(:, j) L(:, j)^T and
+// x -= L(:, j) X(j, :) below row j; the factors come from the lanes that hold
+// them (ds_bpermute). Writes L back into the packed block, X (lower, zero
+// elsewhere, 16 x 16 row-major, leading dimension 17) to sx and, 16 x 16
+// row-major, to xg. Returns the dropped pivots.
+__device__ int leaf16(float* M, int k0, int kb, int m, float* sx, float* xg, int lane) {
+  const int c = lane & 15, rg = lane >> 4;
+  float f[4], x[4];
+#pragma unroll
+  for (int g = 0; g < 4; ++g) {
+    const int r = rg + 4 * g;
+    f[g] = (r < kb && c <= r) ? M[packed_index(k0 + r, k0 + c, m)] : 0.f;
+    x[g] = (r == c && r < kb) ? 1.f : 0.f;
+  }
+  int dropped = 0;
+#pragma unroll
+  for (int j = 0; j < kBlk; ++j) {
+    if (j < kb) {
+      const int jg = j >> 2, jl = j & 3;  // (j, j) sits on lane j + 16 jl, register jg
+      const double dk = static_cast<double>(__shfl(f[jg], j + 16 * jl, 64));
+      const bool drop = !(dk > 1e-13) && isfinite(dk);
+      const float piv = drop ? Prec<float>::drop_pivot : (isfinite(dk) ? static_cast<float>(sqrt(dk)) : 1.f);
+      const float rp = 1.f / piv;
+      dropped += drop ? 1 : 0;
+      if (c == j) {  // column j of L: scaled below the diagonal, the pivot on it, zero above
+#pragma unroll
+        for (int g = 0; g < 4; ++g) {
+          const int r = rg + 4 * g;
+          f[g] = r > j ? f[g] * rp : (r == j ? piv : 0.f);
+        }
+      }
+      if (rg == jl) x[jg] *= rp;  // row j of X
+      float lr[4], t[4];
+#pragma unroll
+      for (int g = 0; g < 4; ++g) lr[g] = __shfl(f[g], j + 16 * rg, 64);  // L(r_g, j)
+#pragma unroll
+      for (int g = 0; g < 4; ++g) t[g] = __shfl(f[g], j + 16 * (c & 3), 64);
+      const int cg = c >> 2;
+      const float lc = cg == 0 ? t[0] : cg == 1 ? t[1] : cg == 2 ? t[2] : t[3];  // L(c, j)
+      const float xr = __shfl(x[jg], c + 16 * jl, 64);                           // X(j, c)
+#pragma unroll
+      for (int g = 0; g < 4; ++g) {
+        const int r = rg + 4 * g;
+        if (r > j) {
+          if (c > j) f[g] -= lr[g] * lc;
+          x[g] -= lr[g] * xr;
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int g = 0; g < 4; ++g) {
+    const int r = rg + 4 * g;
+    const bool low = r < kb && c <= r;
+    if (low) M[packed_index(k0 + r, k0 + c, m)] = f[g];
+    sx[r * (kBlk + 1) + c] = low ? x[g] : 0.f;
+    xg[r * kBlk + c] = low ? x[g] : 0.f;
+  }
+  return dropped;
+}
+
+// Blocked right-looking Cholesky of the FP32 packed lower triangle in LDS,
+// 16 columns a step: (1) wavefront 0 factors and inverts the 16 x 16
+// diagonal block in registers (leaf16); (2) the panel below it is
+// L(I, K) = A(I, K) X(K, K)^T, one 16-row MFMA tile per wavefront; (3) the
+// trailing lower triangle takes the panel's rank-16 update on MFMA
+// (v_mfma_f32_16x16x4f32), one 16 x 16 tile per wavefront at a time. Three
+// barriers a block. The diagonal blocks' inverses go to xg (16 x 16 each,
+// row-major) for invert_lower_blocked. Returns the dropped pivots (counted
+// on wavefront 0).
+__device__ int cholesky_blocked(const Lp& L, float* M, float* sx) {
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int lrow = lane & 15, lk = lane >> 4;
+  const int m = L.m;
+  int dropped = 0;
+  for (int k0 = 0; k0 < m; k0 += kBlk) {
+    const int kb = min(kBlk, m - k0);
+    if (wv == 0) dropped += leaf16(M, k0, kb, m, sx, L.xs + (k0 / kBlk) * kBlk * kBlk, lane);
+    __syncthreads();
+    const int r0 = k0 + kb;  // first row below the block
+    if (r0 >= m) break;      // (only the last block is narrower than kBlk)
+    for (int pt = wv; r0 + kBlk * pt < m; pt += kWaves) {  // (2) rows r0 + 16 pt .. of the panel
+      const int ra = r0 + kBlk * pt + lrow;
+      float av[4], bv[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        av[u] = packed_at(M, ra, k0 + 4 * u + lk, m);  // A(ra, k0 + q)
+        bv[u] = sx[lrow * (kBlk + 1) + 4 * u + lk];    // X(p, q), p = lrow
+      }
+      flt4_t acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int u = 0; u < 4; ++u) acc = Prec<float>::mfma(av[u], bv[u], acc);
+#pragma unroll
+      for (int g = 0; g < 4; ++g) {
+        const int i = r0 + kBlk * pt + Prec<float>::row(lk, g);
+        if (i < m) M[packed_index(i, k0 + lrow, m)] = acc[g];
+      }
+    }
+    __syncthreads();
+    {  // (3) trailing update, tiles (I, J), J <= I, of the rows and columns from r0
+      const int nt = (m - r0 + kBlk - 1) / kBlk;
+      const int tiles = nt * (nt + 1) / 2;
+      for (int t = wv; t < tiles; t += kWaves) {
+        int ti = static_cast<int>((sqrt(8.0 * t + 1.0) - 1.0) * 0.5);
+        while ((ti + 1) * (ti + 2) / 2 <= t) ++ti;
+        while (ti * (ti + 1) / 2 > t) --ti;
+        const int tj = t - ti * (ti + 1) / 2;
+        const int ra = r0 + kBlk * ti + lrow, cb = r0 + kBlk * tj + lrow;
+        float av[4], bv[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const int k = k0 + 4 * u + lk;
+          av[u] = packed_at(M, ra, k, m);
+          bv[u] = packed_at(M, cb, k, m);
+        }
+        flt4_t acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int u = 0; u < 4; ++u) acc = Prec<float>::mfma(av[u], bv[u], acc);
+#pragma unroll
+        for (int g = 0; g < 4; ++g) {
+          const int i = r0 + kBlk * ti + Prec<float>::row(lk, g), j = cb;
+          if (i < m && j < m && i >= j) M[packed_index(i, j, m)] -= acc[g];
+        }
+      }
+    }
+    __syncthreads();
+  }
+  return dropped;
+}
+
+// In place: the FP32 packed factor L in LDS becomes X = L^-1, block column
+// by block column from the last. For block column K: (a) the diagonal
+// block's inverse, formed by cholesky_blocked, is copied in from xg; (b) for
+// each block row I > K (spread over the wavefronts,
+// block rows per wavefront at most kRowsPerWave)
+// T = sum_{J=K+1..I} X(I, J) L(J, K) on MFMA, kept in registers until every
+// wavefront has read L(., K); (c) T goes into the place of
+// L(I, K) and X(I, K) = -T X(K, K) on MFMA.
+__device__ void invert_lower_blocked(const Lp& L, float* M) {
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int lrow = lane & 15, lk = lane >> 4;
+  const int m = L.m;
+  const int nbk = (m + kBlk - 1) / kBlk;
+  for (int K = nbk - 1; K >= 0; --K) {
+    const int k0 = kBlk * K, kb = min(kBlk, m - k0);
+    if (tid < kBlk * kBlk) {  // (a)
+      const int r = tid / kBlk, c = tid % kBlk;
+      if (r < kb && c <= r) M[packed_index(k0 + r, k0 + c, m)] = L.xs[K * kBlk * kBlk + tid];
+    }
+    __syncthreads();
+    flt4_t acc[kRowsPerWave];  // (b) block rows K + 1 + wv + kWaves q of this wavefront
+#pragma unroll
+    for (int q = 0; q < kRowsPerWave; ++q) {
+      acc[q] = flt4_t{0.f, 0.f, 0.f, 0.f};
+      const int I = K + 1 + wv + kWaves * q;
+      if (I >= nbk) continue;
+      const int r0 = kBlk * I;
+      for (int J = K + 1; J <= I; ++J) {
+        const int j0 = kBlk * J;
+        float av[4], bv[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const int kk = j0 + 4 * u + lk;
+          av[u] = packed_at(M, r0 + lrow, kk, m);  // X(I, J): zero above the diagonal
+          bv[u] = packed_at(M, kk, k0 + lrow, m);  // L(J, K)
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) acc[q] = Prec<float>::mfma(av[u], bv[u], acc[q]);
+      }
+    }
+    __syncthreads();  // every L(., K) read
+#pragma unroll
+    for (int q = 0; q < kRowsPerWave; ++q) {  // (c)
+      const int I = K + 1 + wv + kWaves * q;
+      if (I >= nbk) continue;
+      const int r0 = kBlk * I;
+#pragma unroll
+      for (int g = 0; g < 4; ++g) {
+        const int i = r0 + Prec<float>::row(lk, g), j = k0 + lrow;
+        if (i < m && j < k0 + kb) M[packed_index(i, j, m)] = acc[q][g];
+      }
+      wave_sync();
+      float av[4], bv[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int kk = k0 + 4 * u + lk;
+        av[u] = (kk < k0 + kb) ? packed_at(M, r0 + lrow, kk, m) : 0.f;                // T(I, K)
+        bv[u] = (kk < k0 + kb && lrow < kb) ? packed_at(M, kk, k0 + lrow, m) : 0.f;  // X(K, K)
+      }
+      flt4_t x = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int u = 0; u < 4; ++u) x = Prec<float>::mfma(av[u], bv[u], x);
+      wave_sync();
+#pragma unroll
+      for (int g = 0; g < 4; ++g) {
+        const int i = r0 + Prec<float>::row(lk, g), j = k0 + lrow;
+        if (i < m && j < k0 + kb) M[packed_index(i, j, m)] = -x[g];
+      }
+    }
+    __syncthreads();
+  }
+}
+
 // q = A D A^T p (FP64, matrix-free); p, q global (m).
 __device__ void normal_matvec(const Lp& L, const double* p, double* q, double* stage) {
   gemv_t(L, p, L.t, stage);
@@ -474,6 +788,7 @@ __global__ __launch_bounds__(kThreads) void batch_ipm_kernel(const BatchMatrix* 
   float* M = reinterpret_cast<float*>(red + 64);             // packed lower triangle
   __shared__ int s_lp;
   __shared__ Lp s_L;
+  __shared__ float s_xkk[kBlk * (kBlk + 1)];  // inverse of the diagonal block being factored
   const int tid = threadIdx.x;
   for (;;) {
     __syncthreads();
@@ -492,6 +807,8 @@ __global__ __launch_bounds__(kThreads) void batch_ipm_kernel(const BatchMatrix* 
       W.n = mat.n;
       W.A = mat.a64;
       W.Af = mat.a32;
+      W.Ar = mat.a64r;
+      W.ldr = mat.ldr;
       W.b = b_all + dsc.b_off;
       W.c = c_all + dsc.x_off;
       W.u = u_all + dsc.x_off;
@@ -508,6 +825,11 @@ __global__ __launch_bounds__(kThreads) void batch_ipm_kernel(const BatchMatrix* 
         wsp += W.m;
       }
       W.M64 = wsp;  // m (m + 1) / 2
+      W.xs = reinterpret_cast<float*>(wsp + static_cast<long long>(W.m) * (W.m + 1) / 2);  // 256 per block
+      {  // ldr floats, 16-byte aligned for vector loads (the workspace holds 16 bytes of slack)
+        const std::uintptr_t e = reinterpret_cast<std::uintptr_t>(W.xs + kBlk * kBlk * ((W.m + kBlk - 1) / kBlk));
+        W.sd = reinterpret_cast<float*>((e + 15) & ~static_cast<std::uintptr_t>(15));
+      }
     }
     __syncthreads();
     const Lp& L = s_L;
@@ -656,14 +978,14 @@ __global__ __launch_bounds__(kThreads) void batch_ipm_kernel(const BatchMatrix* 
       // the primal feasibility of a nearly solved LP).
       for (;;) {
         if (fp64) assemble(L, L.M64, 0.0);
-        else assemble(L, M, 0.0);
+        else assemble32(L, M, 0.0);
         t0 = clock64();
         note(it, 10, static_cast<double>(t0 - t1));
-        note(it, 15, fp64 ? cholesky(L, L.M64) : cholesky(L, M));
+        note(it, 15, fp64 ? cholesky(L, L.M64) : cholesky_blocked(L, M, s_xkk));
         t1 = clock64();
         note(it, 11, static_cast<double>(t1 - t0));
         if (fp64) invert_lower(L, L.M64, stage);
-        else invert_lower(L, M, stage);
+        else invert_lower_blocked(L, M);
         // inexact Newton while mu is large: CG relative residual max(pcg_tol, min(1e-6, 1e-3 mu))
         const double ptol = fp64 ? o.pcg_tol : fmax(o.pcg_tol, fmin(1e-6, 1e-3 * mu));
         t0 = clock64();
@@ -840,14 +1162,23 @@ int sov_batch_add_matrix(sov_batch* h, int32_t m, int32_t n, const double* a_col
     return SOV_ERR_ARG;
   }
   const std::size_t cnt = static_cast<std::size_t>(m) * n;
-  std::vector<float> af(cnt);
-  for (std::size_t k = 0; k < cnt; ++k) af[k] = static_cast<float>(a_colmajor[k]);
+  const int ldr = row_ld(n);
+  const std::size_t cntr = static_cast<std::size_t>(m) * ldr;
+  std::vector<float> af(cntr, 0.f);
+  for (int j = 0; j < n; ++j)
+    for (int i = 0; i < m; ++i)
+      af[static_cast<std::size_t>(i) * ldr + j] = static_cast<float>(a_colmajor[i + static_cast<std::size_t>(j) * m]);
+  std::vector<double> ar(cntr, 0.0);
+  for (int j = 0; j < n; ++j)
+    for (int i = 0; i < m; ++i) ar[static_cast<std::size_t>(i) * ldr + j] = a_colmajor[i + static_cast<std::size_t>(j) * m];
   double* a64 = dev_alloc<double>(h, cnt);
-  float* a32 = dev_alloc<float>(h, cnt);
-  if (!a64 || !a32) return SOV_ERR_NOMEM;
+  float* a32 = dev_alloc<float>(h, cntr);
+  double* a64r = dev_alloc<double>(h, cntr);
+  if (!a64 || !a32 || !a64r) return SOV_ERR_NOMEM;
   SOV_HIP_TRY(hipMemcpy(a64, a_colmajor, cnt * sizeof(double), hipMemcpyHostToDevice));
-  SOV_HIP_TRY(hipMemcpy(a32, af.data(), cnt * sizeof(float), hipMemcpyHostToDevice));
-  h->mats.push_back({m, n, a64, a32});
+  SOV_HIP_TRY(hipMemcpy(a32, af.data(), cntr * sizeof(float), hipMemcpyHostToDevice));
+  SOV_HIP_TRY(hipMemcpy(a64r, ar.data(), cntr * sizeof(double), hipMemcpyHostToDevice));
+  h->mats.push_back({m, n, a64, a32, ldr, a64r});
   *id = static_cast<int32_t>(h->mats.size() - 1);
   return SOV_OK;
 }
@@ -874,7 +1205,11 @@ int sov_batch_solve(sov_batch* h, int32_t count, const int32_t* matrix_id, const
     descs[k] = {id, 0, b_off[k], x_off[k], ws_total};
     b_total = std::max<long long>(b_total, b_off[k] + mt.m);
     x_total = std::max<long long>(x_total, x_off[k] + mt.n);
-    ws_total += 16LL * mt.n + 9LL * mt.m + static_cast<long long>(mt.m) * (mt.m + 1) / 2;
+    // 16 n-vectors, 9 m-vectors, the packed FP64 matrix, 128 doubles (256
+    // floats) per 16-row block for the diagonal-block inverses, and ldr
+    // floats for sqrt(D)
+    ws_total += 16LL * mt.n + 9LL * mt.m + static_cast<long long>(mt.m) * (mt.m + 1) / 2 + 128LL * ((mt.m + 15) / 16) +
+                mt.ldr / 2 + 2;
   }
   TmpBuffers tmp;  // per-solve device buffers, freed at the end of the call
   auto* d_mats = tmp.get<BatchMatrix>(h->mats.size());

@@ -160,6 +160,47 @@ def test_batch_uneven_sizes_match_oracle(gpu_api):
 RULES = ["strong", "full_strong", "pseudocost", "most_fractional"]
 
 
+def test_batch_forced_fp64_switch_matches_oracle(gpu_api):
+    """refine_max = 1 makes the first FP32-preconditioned solve that needs a
+    CG iteration switch its LP to FP64 factorizations in global memory
+    (the iteration redone): every LP must take that path (fp64_from >= 0)
+    and still match the oracle's objective to 1e-8."""
+    bs = MI.BatchLpSolver()
+    lps, models = [], []
+    for (n, m, seed) in ((40, 10, 1), (70, 25, 2), (120, 50, 3)):
+        mk = P.multi_knapsack(n, m, seed=seed)
+        W = mk.dense()
+        A = np.hstack([W / W.max(axis=1, keepdims=True), np.eye(m)])
+        b = mk.rhs / W.max(axis=1)
+        c = np.concatenate([-mk.obj / mk.obj.max(), np.zeros(m)])
+        mid = bs.add_matrix(A)
+        for k in range(2):
+            u = np.concatenate([np.ones(n), np.full(m, 1e30)])
+            u[k] = 0.0
+            lps.append((mid, b, c, u))
+            models.append((A, b, c, u))
+    say("batch forced FP64 switch, 6 LPs")
+    opts = gpu_api.InteriorPointOptions(feasibility_tol=1e-9, optimality_tol=1e-9, refine_max=1)
+    res, xs = bs.solve(lps, options=opts, want_x=True)
+    for r, (A, b, c, u), x in zip(res, models, xs):
+        say(f"  status {r['status']} it {r['iterations']} fp64_from {r['fp64_from']} obj {r['objective']:.12g}")
+        assert r["status"] == 0 and r["fp64_from"] >= 0
+        m, n = A.shape
+        cols = [list(np.nonzero(A[:, j])[0]) for j in range(n)]
+        ptr, rows, vals = P.csc_from_columns(cols, [list(A[cl, j]) for j, cl in enumerate(cols)], m)
+        gm = P.LpModel(n_vars=n, n_rows=m, col_ptr=ptr, row_idx=rows, values=vals, obj=c, lb=np.zeros(n), ub=u,
+                       row_sense=b"E" * m, rhs=b)
+        ref = O.lp_solve(gm)
+        assert ref["status"] == 0
+        assert rel(r["objective"], ref["objective"]) <= 1e-8
+        assert np.max(np.abs(A @ x - b)) <= 1e-7
+    # and the default path does not switch on these well-conditioned LPs
+    res2, _ = bs.solve(lps)
+    assert all(r["status"] == 0 for r in res2)
+    for r, r2 in zip(res, res2):
+        assert rel(r["objective"], r2["objective"]) <= 1e-8
+
+
 @pytest.mark.parametrize("rule", RULES)
 @pytest.mark.parametrize("name", ["multi_knapsack_18x3", "multi_knapsack_24x4"])
 def test_bnb_reference_fixtures(gpu_api, name, rule):
