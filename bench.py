@@ -79,7 +79,8 @@ def tuned_shapes(lib):
                    "panel_ms": {"64x64": tm[off + 2], "128x64": tm[off + 3], "128x128": tm[off + 4],
                                 "32x32": tm[off + 5]}}
     return out, sh
-NT = 12  # slots of sov_kernel_times
+NT = 24  # slots of sov_kernel_times
+HBM_PEAK = 8000.0  # GB/s, MI355X HBM3E
 
 
 def log(msg):
@@ -123,8 +124,9 @@ def timed_solves(ipm, opts, steps, warmup, lib):
 
 
 def kernel_ms(kt):
-    return {"syrk_f64": kt[0], "syrk_f32": kt[9], "panel_chain": kt[3], "strip_update": kt[7],
-            "super_inverse": kt[8], "triangular_solves": kt[4], "assembly": kt[5]}
+    return {"syrk_f64": kt[0], "syrk_f32": kt[9], "panel_chain": kt[3], "panel_kernel_f64": kt[12],
+            "panel_kernel_f32": kt[15], "strip_update": kt[7], "super_inverse": kt[8], "triangular_solves": kt[4],
+            "assembly": kt[5]}
 
 
 def load_traffic():
@@ -136,6 +138,8 @@ def load_traffic():
 
 
 def roofline(kt, dtype, traffic, shape=0):
+    """The bulk SYRK of one precision: algorithmic flops M(M+1)K per launch
+    over its event-timed launch duration."""
     ms, launches, flops = (kt[0], kt[1], kt[2]) if dtype == "f64" else (kt[9], kt[10], kt[11])
     if ms <= 0 or launches <= 0:
         return None
@@ -148,7 +152,94 @@ def roofline(kt, dtype, traffic, shape=0):
             "kernel": name + " (bulk SYRK trailing update)",
             "launches": int(launches), "avg_launch_us": ms * 1e3 / launches,
             "flops_per_launch": flops / launches,
+            "traffic_over_algorithmic_bytes": (tr["bytes_per_launch"] / tr["algorithmic_bytes_per_launch"]
+                                               if tr and tr.get("algorithmic_bytes_per_launch") else None),
             "traffic_source": tr.get("source") if tr else None}
+
+
+def panel_roofline(kt, dtype, traffic):
+    """The diagonal-block (panel) kernel: one workgroup factors and inverts a
+    128 x 128 block; algorithmic work 2 nb^3 / 3 flop per launch (factor and
+    inverse) over its event-timed launch duration."""
+    ms, launches, flops = (kt[12], kt[13], kt[14]) if dtype == "f64" else (kt[15], kt[16], kt[17])
+    if ms <= 0 or launches <= 0:
+        return None
+    achieved = flops / (ms * 1e-3) / 1e12
+    name = f"potrf_panel_kernel<{'double' if dtype == 'f64' else 'float'}>"
+    tr = traffic.get(name)
+    return {"bound": "mfma", "achieved": achieved, "peak": MFMA_PEAK[dtype], "unit": "TFLOP/s",
+            "frac": achieved / MFMA_PEAK[dtype], "traffic": tr["bytes_per_launch"] if tr else None,
+            "kernel": name + " (diagonal block: factor + inverse, one workgroup, on the critical path)",
+            "launches": int(launches), "avg_launch_us": ms * 1e3 / launches, "flops_per_launch": flops / launches,
+            "traffic_source": tr.get("source") if tr else None}
+
+
+def dominant_roofline(kt, traffic, shape_ids):
+    """Roofline of the event-timed kernel with the largest time in the timed
+    region (the panel kernel and the bulk SYRK of each precision are timed
+    there by their own launch events), with the others as secondary."""
+    entries = {
+        "panel_f32": (kt[15], panel_roofline(kt, "f32", traffic)),
+        "panel_f64": (kt[12], panel_roofline(kt, "f64", traffic)),
+        "syrk_f32": (kt[9], roofline(kt, "f32", traffic, shape_ids[2])),
+        "syrk_f64": (kt[0], roofline(kt, "f64", traffic, shape_ids[0])),
+    }
+    live = {k: v for k, v in entries.items() if v[1] is not None}
+    top = max(live, key=lambda k: live[k][0])
+    total = sum(v[0] for v in live.values())
+    main = dict(live[top][1])
+    main["timed_region_ms"] = live[top][0]
+    others = []
+    for k, (ms, e) in sorted(live.items(), key=lambda kv: -kv[1][0]):
+        if k != top:
+            e = dict(e)
+            e["timed_region_ms"] = ms
+            others.append(e)
+    main["share_of_event_timed_kernels"] = live[top][0] / total if total > 0 else None
+    return main, others
+
+
+def spmv_roofline(lib, mat, m, n, reps=20):
+    """Standalone CSR SpMV (y = A x and y = A' x) on the resident matrix:
+    algorithmic bytes 12 nnz + 4 (rows + 1) + 8 (m + n) per product over the
+    kernel's event-timed duration."""
+    x = np.linspace(0.5, 1.5, n)
+    y = np.linspace(-1.0, 1.0, m)
+    out = {}
+    for tr, vin, vout in ((0, x, np.zeros(m)), (1, y, np.zeros(n))):
+        lib.sov_spmv(mat.handle, tr, K_pf(vin), K_pf(vout))  # warm-up
+        lib.sov_sync()
+        kernel_times(lib)
+        lib.sov_set_kernel_timing(1)
+        for _ in range(reps):
+            lib.sov_spmv(mat.handle, tr, K_pf(vin), K_pf(vout))
+        lib.sov_sync()
+        kt = kernel_times(lib)
+        lib.sov_set_kernel_timing(0)
+        ms, launches, nbytes = kt[18], kt[19], kt[20]
+        gbps = nbytes / (ms * 1e-3) / 1e9
+        out["A'x" if tr else "Ax"] = {"bound": "hbm", "achieved": gbps, "peak": HBM_PEAK, "unit": "GB/s",
+                                      "frac": gbps / HBM_PEAK, "avg_launch_us": ms * 1e3 / launches,
+                                      "bytes_per_launch": nbytes / launches, "launches": int(launches)}
+    return out
+
+
+def solve_roofline(kt, dtype):
+    """The blocked triangular solves (GEMV steps): the factor's lower
+    triangle read by the forward and the backward pass, m (m + 1) elements
+    per solve pair, over the pairs' event-timed duration (one instrumented
+    solve, every kernel class timed)."""
+    ms, pairs, nbytes = kt[4], kt[21], kt[22]
+    if ms <= 0 or pairs <= 0:
+        return None
+    gbps = nbytes / (ms * 1e-3) / 1e9
+    return {"bound": "hbm", "achieved": gbps, "peak": HBM_PEAK, "unit": "GB/s", "frac": gbps / HBM_PEAK,
+            "solve_pairs": int(pairs), "avg_pair_us": ms * 1e3 / pairs, "bytes_per_pair": nbytes / pairs,
+            "note": f"{dtype} path; FP32 and FP64 solves together in the mixed path"}
+
+
+def K_pf(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
 
 
 def cpu_baseline(sizes=(1024, 2048)):
@@ -196,6 +287,17 @@ def branching_leg(lib):
     children = 2 * len(cb)
     diff = max([abs(a[f"obj_{s}"] - b[f"obj_{s}"]) / max(1.0, abs(b[f"obj_{s}"]))
                 for a, b in zip(cb, cp) for s in ("down", "up")] or [0.0])
+    # algorithmic flops of the batched kernel: per child-LP iteration the
+    # SYRK n m (m + 1), the Cholesky m^3 / 3 and the GEMVs 12 m n
+    mi, ni = model.n_rows, model.n_vars + model.n_rows
+    per_it = ni * mi * (mi + 1) + mi ** 3 / 3 + 12 * mi * ni
+    iters = sum(c["iters_down"] + c["iters_up"] for c in cb)
+    bflops = per_it * iters
+    out["roofline_batched_kernel"] = {
+        "bound": "mfma", "achieved": bflops / (kms * 1e-3) / 1e12, "peak": MFMA_PEAK["f32"], "unit": "TFLOP/s",
+        "frac": bflops / (kms * 1e-3) / 1e12 / MFMA_PEAK["f32"], "kernel": "batch_ipm_kernel (one launch, 128 LPs)",
+        "flops_per_child_iteration": per_it, "child_iterations": iters, "kernel_ms": kms,
+        "note": "FP32 MFMA SYRK and LDS Cholesky per child; 128 children occupy 128 of 256 CUs"}
     out.update({
         "candidates": len(cb), "child_lps": children,
         "batched_wall_s": wall_b, "batched_kernel_ms": kms, "batched_lp_solves_per_s": children / wall_b,
@@ -401,13 +503,27 @@ def main():
         runs[path] = timed_solves(ipm, opts, args.steps, args.warmup, lib)
         log(f"kernel ms ({path}, one instrumented solve): " +
             " ".join(f"{k} {v:.1f}" for k, v in kernel_ms(runs[path][3]).items()))
+    spmv = spmv_roofline(lib, ipm.A, lp.m, lp.n)
+    log("spmv " + " ".join(f"{k} {v['achieved']:.0f} GB/s ({v['avg_launch_us']:.1f} us)" for k, v in spmv.items()))
     ipm.close()
+    # end to end: upload and plan (the reference's NormalEquations constructor
+    # runs inside every solve_ipm) plus one solve, per path
+    end_to_end = {}
+    for path in paths:
+        opts = api.InteriorPointOptions(precision=path, **tol)
+        lib.sov_sync()
+        t = time.perf_counter()
+        fresh = api.ResidentIpm(lp)
+        r = fresh.solve(opts)
+        lib.sov_sync()
+        end_to_end[path] = time.perf_counter() - t
+        fresh.close()
+        log(f"end to end ({path}): setup + solve {end_to_end[path]:.3f}s, status {r['status_name']}")
     main_path = "mixed" if "mixed" in runs else paths[0]
     res, wall, kt, _ = runs[main_path]
     traffic = load_traffic()
     shapes_info, shape_ids = tuned_shapes(lib)
-    # dominant MFMA kernel of the measured path: the SYRK precision with the most time
-    dom = "f32" if kt[9] >= kt[0] else "f64"
+    dominant, secondary = dominant_roofline(kt, traffic, shape_ids)
     line = {
         "metric": METRIC,
         "value": wall / args.steps,
@@ -427,8 +543,12 @@ def main():
         "factorizations": {p: {"fp32": rr[0][-1]["fp32_factorizations"], "total": rr[0][-1]["factorizations"]}
                            for p, rr in runs.items()},
         "resident_setup_s": setup_s,
+        "end_to_end_s_incl_setup": end_to_end,
         "kernel_ms_one_instrumented_solve": {p: kernel_ms(rr[3]) for p, rr in runs.items()},
-        "roofline": roofline(kt, dom, traffic, shape_ids[0 if dom == "f64" else 2]),
+        "roofline": dominant,
+        "roofline_secondary": secondary,
+        "roofline_spmv": spmv,
+        "roofline_triangular_solves": {p: solve_roofline(rr[3], p) for p, rr in runs.items()},
         "autotuned_tiles": shapes_info,
     }
     if "mixed" in runs and "fp64" in runs:
@@ -441,7 +561,7 @@ def main():
             "objective_rel_diff_mixed_vs_fp64": abs(o_mx - o_64) / max(1.0, abs(o_64)),
             "pcg_iterations_mixed": runs["mixed"][0][-1]["refine_iterations"],
             "max_solve_residual_mixed": runs["mixed"][0][-1]["max_solve_residual"],
-            "roofline_fp64_path": roofline(runs["fp64"][2], "f64", traffic, shape_ids[0]),
+            "roofline_fp64_path": dominant_roofline(runs["fp64"][2], traffic, shape_ids)[0],
         })
     if not args.no_branching:
         line["strong_branching"] = branching_leg(lib)

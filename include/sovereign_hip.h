@@ -62,12 +62,13 @@ int sov_sync(void);
 /* The library's HIP stream (hipStream_t), for callers that time kernels. */
 void* sov_stream(void);
 
-/* Kernel timing: with timing on (1), the factorization and solve kernels are
- * bracketed by HIP events on the streams they run on (recording does not
- * wait; the totals are collected after one stream synchronisation); 2 times
- * the bulk trailing-update SYRK only (slots 0-2 and 9-11), which keeps the
- * event traffic off the critical path of the factorization.
- * sov_kernel_times fills out[0..11] with the totals since the last call and
+/* Kernel timing: with timing on (1), the factorization, solve and SpMV
+ * kernels are bracketed by HIP events on the streams they run on (recording
+ * does not wait; the totals are collected after one stream
+ * synchronisation); 2 times only the kernels whose own launches record the
+ * events (the bulk trailing-update SYRK and the diagonal-block kernel),
+ * which keeps event traffic off the factorization's critical path.
+ * sov_kernel_times fills out[0..23] with the totals since the last call and
  * resets them:
  *   0 FP64 trailing-update SYRK ms (the bulk update of each panel, MFMA),
  *   1 its launches, 2 its flops (algorithmic: lower triangle, M(M+1)K),
@@ -75,10 +76,17 @@ void* sov_stream(void);
  *   4 triangular-solve ms, 5 normal-equation assembly ms,
  *   6 panel-chain launches, 7 look-ahead strip update ms,
  *   8 super-block inverse ms,
- *   9 FP32 trailing-update SYRK ms, 10 its launches, 11 its flops.
+ *   9 FP32 trailing-update SYRK ms, 10 its launches, 11 its flops,
+ *   12 FP64 diagonal-block kernel ms, 13 its launches, 14 its flops
+ *      (2 nb^3 / 3 per launch: the block's factor and its inverse),
+ *   15-17 the same for the FP32 diagonal-block kernel,
+ *   18 standalone SpMV kernel ms, 19 its launches, 20 its algorithmic bytes
+ *      (12 nnz + 4 (rows + 1) + 8 (m + n) per product),
+ *   21 triangular solve pairs, 22 their algorithmic bytes (the factor's
+ *      lower triangle read twice, m (m + 1) elements per pair), 23 unused.
  * No counterpart in the reference, which times nothing on the device. */
 void sov_set_kernel_timing(int32_t on);
-int sov_kernel_times(double* out12);
+int sov_kernel_times(double* out24);
 
 /* MFMA tile shapes chosen by the factorization's autotuner (0: 64x64,
  * 1: 128x64, 2: 128x128, 3: 32x32; -1 before the first factorization in

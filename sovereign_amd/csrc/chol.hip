@@ -607,12 +607,6 @@ bool configure_lds() {
 // Event pairs recorded around kernels while timing is on. Recording does
 // not wait; the elapsed times are collected (after one stream sync) by
 // sov_kernel_times, so timing can stay on inside a timed region.
-struct TimedPair {
-  hipEvent_t a = nullptr, b = nullptr;
-  int kind = 0;
-  double flops = 0.0;
-  int launches = 0;
-};
 struct EventPool {
   std::vector<TimedPair> pairs;
   std::size_t used = 0;
@@ -635,7 +629,11 @@ EventPool& pool() {
 // timing is on for `kind` (nullptr otherwise).
 TimedPair* timed_kernel(int kind) {
   const int level = g_timing.load();
-  if (level == 0 || (level == 2 && kind != kTimeSyrk && kind != kTimeSyrk32)) return nullptr;
+  // level 2 keeps to the kernels whose launches record their own events:
+  // the bulk SYRK and the diagonal-block kernel
+  if (level == 0 || (level == 2 && kind != kTimeSyrk && kind != kTimeSyrk32 && kind != kTimePotrf64 &&
+                     kind != kTimePotrf32))
+    return nullptr;
   TimedPair* p = pool().next();
   if (!p) return nullptr;
   p->kind = kind;
@@ -692,8 +690,22 @@ void collect_kernel_times(hipStream_t s) {
       t.syrk32_ms += ms;
       t.syrk32_launches += e.launches;
       t.syrk32_flops += e.flops;
+    } else if (e.kind == kTimePotrf64) {
+      t.potrf64_ms += ms;
+      t.potrf64_launches += e.launches;
+      t.potrf64_flops += e.flops;
+    } else if (e.kind == kTimePotrf32) {
+      t.potrf32_ms += ms;
+      t.potrf32_launches += e.launches;
+      t.potrf32_flops += e.flops;
+    } else if (e.kind == kTimeSpmv) {
+      t.spmv_ms += ms;
+      t.spmv_launches += e.launches;
+      t.spmv_bytes += e.flops;
     } else {
       t.solve_ms += ms;
+      t.solve_pairs += 1.0;
+      t.solve_bytes += e.flops;
     }
   }
   p.used = 0;
@@ -851,7 +863,15 @@ int DenseCholesky<T>::factor(T* a, int m, const double* diag0, hipStream_t s) {
               static_cast<long long>(k0 % kSuper) * (kSuper + 1);
     T* L21 = panel_ + (p % kPanelBuffers) * pstride;
     TimedPair* tp = timed_begin(s, kTimePanel);
-    potrf_panel_kernel<T><<<1, kPanelThreads, panel_lds_bytes<T>(), s>>>(a, lda, k0, nb, diag0, flags_, pinv, kSuper);
+    TimedPair* tk = timed_kernel(sizeof(T) == 8 ? kTimePotrf64 : kTimePotrf32);
+    if (tk) {  // the launch records its own start and stop events
+      tk->launches = 1;
+      tk->flops = 2.0 * nb * static_cast<double>(nb) * nb / 3.0;  // factor nb^3/3 + inverse nb^3/3
+      hipExtLaunchKernelGGL(potrf_panel_kernel<T>, dim3(1), dim3(kPanelThreads), panel_lds_bytes<T>(), s, tk->a, tk->b,
+                            0, a, lda, k0, nb, diag0, flags_, pinv, kSuper);
+    } else {
+      potrf_panel_kernel<T><<<1, kPanelThreads, panel_lds_bytes<T>(), s>>>(a, lda, k0, nb, diag0, flags_, pinv, kSuper);
+    }
     SOV_LAUNCH_CHECK();
     if (below > 0) {
       // L21 = A21 inv(L11)^T on MFMA, out of place (the updates read it
@@ -990,7 +1010,8 @@ int DenseCholesky<T>::solve(const T* a, int m, double* x, double* /*work*/, hipS
     SOV_LAUNCH_CHECK();
     return SOV_OK;
   });
-  timed_end(tp, s, 4 * nsb, 0.0);
+  // algorithmic bytes: the factor's lower triangle read by the forward and the backward pass
+  timed_end(tp, s, 4 * nsb, static_cast<double>(m) * (m + 1) * sizeof(T));
   return rc;
 }
 
@@ -1024,10 +1045,12 @@ int sov_kernel_times(double* out) {
   if (!out) return SOV_ERR_ARG;
   if (sov::device_ready()) sov::collect_kernel_times(sov::stream());
   sov::KernelTimes& t = sov::kernel_times();
-  const double v[12] = {t.syrk_ms,        t.syrk_launches, t.syrk_flops, t.panel_ms,    t.solve_ms,
-                        t.assemble_ms,    t.panel_launches, t.strip_ms, t.inverse_ms, t.syrk32_ms,
-                        t.syrk32_launches, t.syrk32_flops};
-  for (int i = 0; i < 12; ++i) out[i] = v[i];
+  const double v[24] = {t.syrk_ms,        t.syrk_launches,    t.syrk_flops,      t.panel_ms,       t.solve_ms,
+                        t.assemble_ms,    t.panel_launches,   t.strip_ms,        t.inverse_ms,     t.syrk32_ms,
+                        t.syrk32_launches, t.syrk32_flops,    t.potrf64_ms,      t.potrf64_launches, t.potrf64_flops,
+                        t.potrf32_ms,     t.potrf32_launches, t.potrf32_flops,   t.spmv_ms,        t.spmv_launches,
+                        t.spmv_bytes,     t.solve_pairs,      t.solve_bytes,     0.0};
+  for (int i = 0; i < 24; ++i) out[i] = v[i];
   t = sov::KernelTimes{};
   return SOV_OK;
 }

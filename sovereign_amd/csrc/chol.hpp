@@ -64,6 +64,11 @@ struct KernelTimes {
   double panel_ms = 0.0, solve_ms = 0.0, assemble_ms = 0.0, panel_launches = 0.0;
   double strip_ms = 0.0, inverse_ms = 0.0;
   double syrk32_ms = 0.0, syrk32_launches = 0.0, syrk32_flops = 0.0;  // FP32 factorizations' SYRK
+  // the diagonal-block (panel) kernel alone, per precision: time, launches, flops (factor + inverse)
+  double potrf64_ms = 0.0, potrf64_launches = 0.0, potrf64_flops = 0.0;
+  double potrf32_ms = 0.0, potrf32_launches = 0.0, potrf32_flops = 0.0;
+  double spmv_ms = 0.0, spmv_launches = 0.0, spmv_bytes = 0.0;  // standalone CSR SpMV launches
+  double solve_pairs = 0.0, solve_bytes = 0.0;                  // triangular solve pairs, factor bytes read
 };
 void set_kernel_timing(int level);
 bool kernel_timing();
@@ -78,9 +83,23 @@ enum TimedKind {
   kTimeStrip = 3,
   kTimeInverse = 4,
   kTimeAssemble = 5,
-  kTimeSyrk32 = 6  // FP32 bulk trailing update
+  kTimeSyrk32 = 6,   // FP32 bulk trailing update
+  kTimePotrf64 = 7,  // FP64 diagonal-block kernel
+  kTimePotrf32 = 8,  // FP32 diagonal-block kernel
+  kTimeSpmv = 9      // standalone CSR SpMV
 };
-struct TimedPair;
+// Event pair of one timed launch (or a run of launches), with its launch
+// count and its algorithmic work (flops, or bytes for the memory-bound
+// kinds).
+struct TimedPair {
+  hipEvent_t a = nullptr, b = nullptr;
+  int kind = 0;
+  double flops = 0.0;
+  int launches = 0;
+};
+// A pair for a kernel launch that records both events itself
+// (hipExtLaunchKernel), or nullptr when timing is off for `kind`.
+TimedPair* timed_kernel(int kind);
 TimedPair* timed_begin(hipStream_t s, int kind);
 void timed_end(TimedPair* p, hipStream_t s, int launches, double flops);
 

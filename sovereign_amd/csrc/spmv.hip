@@ -6,8 +6,11 @@
 // cooperate per output so loads are coalesced, and the device-vector entry
 // point lets the interior-point loop keep every vector resident.
 
+#include "chol.hpp"
 #include "internal.hpp"
 #include "spmv_device.hpp"
+
+#include <hip/hip_ext.h>
 
 #include <algorithm>
 #include <cstring>
@@ -129,9 +132,20 @@ int spmv_launch(const sov_mat* a, bool transpose, const double* x, double* y, hi
   const int32_t* idx = transpose ? a->c_row : a->r_col;
   const double* val = transpose ? a->c_val : a->r_val;
   const int g = transpose ? a->group_cols : a->group_rows;
+  TimedPair* tk = timed_kernel(kTimeSpmv);
+  if (tk) {  // algorithmic bytes: values and indices, row pointers, x and y once each
+    const long long nnz = a->nnz;
+    const int in = transpose ? a->nrows : a->ncols;
+    tk->launches = 1;
+    tk->flops = 12.0 * nnz + 4.0 * (out + 1) + 8.0 * (in + out);
+  }
   SOV_DISPATCH_GROUP(g, {
     const unsigned grid = blocks_for(static_cast<long long>(out) * G, kBlock);
-    csr_spmv_kernel<G><<<grid, kBlock, 0, s>>>(out, ptr, idx, val, x, y);
+    if (tk)
+      hipExtLaunchKernelGGL(csr_spmv_kernel<G>, dim3(grid), dim3(kBlock), 0, s, tk->a, tk->b, 0, out, ptr, idx, val, x,
+                            y);
+    else
+      csr_spmv_kernel<G><<<grid, kBlock, 0, s>>>(out, ptr, idx, val, x, y);
   });
   SOV_LAUNCH_CHECK();
   return SOV_OK;
