@@ -197,7 +197,8 @@ __device__ void gemv_n(const Lp& L, const double* vec, const double* base, doubl
 // and sqrt(D) in FP32, both read 16 bytes a lane: lane group g takes
 // k = k0 + 4 g .. k0 + 4 g + 3, the same k for both operands), S applied to
 // each tile's sums. sc comes from a row-wise FP32 sum of the same operands,
-// one wavefront a row. Two 16-column chunks are loaded ahead of their MFMAs.
+// one wavefront a row. Each wavefront takes up to four tiles of one block
+// row at a time; two 16-column chunks are loaded ahead of their MFMAs.
 __device__ void assemble32(const Lp& L, float* M, double reg) {
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   const int lrow = lane & 15, lk = lane >> 4;
@@ -217,41 +218,74 @@ __device__ void assemble32(const Lp& L, float* M, double reg) {
     if (lane == 0) L.sc[i] = dii + reg > 0.0 ? 1.0 / sqrt(dii + reg) : 1.0;
   }
   __syncthreads();
+  // Work items: block row I with up to four of its tiles (I, J0 .. J0 + 3),
+  // J0 + 3 <= I; the row block's operands are loaded and scaled once per
+  // k-chunk for all of them.
   const int nt = (m + 15) / 16;
-  const int tiles = nt * (nt + 1) / 2;
+  int nitems = 0;
+  for (int I = 0; I < nt; ++I) nitems += I / 4 + 1;
   const flt4_t zero = {0.f, 0.f, 0.f, 0.f};
-  for (int t = wv; t < tiles; t += kWaves) {
-    int ti = static_cast<int>((sqrt(8.0 * t + 1.0) - 1.0) * 0.5);
-    while ((ti + 1) * (ti + 2) / 2 <= t) ++ti;
-    while (ti * (ti + 1) / 2 > t) --ti;
-    const int tj = t - ti * (ti + 1) / 2;
-    const int r0 = ti * 16, c0 = tj * 16;
-    const int ra = r0 + lrow, cb = c0 + lrow;
-    const float* pa = L.Af + static_cast<long long>(ra < m ? ra : 0) * ldr + 4 * lk;
-    const float* pb = L.Af + static_cast<long long>(cb < m ? cb : 0) * ldr + 4 * lk;
-    const float* pd = L.sd + 4 * lk;
-    const bool va = ra < m, vb = cb < m;
-    flt4_t acc = zero;
-    for (int k0 = 0; k0 < ldr; k0 += 32) {  // ldr is a multiple of 32
-      const flt4_t a0 = va ? *reinterpret_cast<const flt4_t*>(pa + k0) : zero;
-      const flt4_t b0 = vb ? *reinterpret_cast<const flt4_t*>(pb + k0) : zero;
-      const flt4_t d0 = *reinterpret_cast<const flt4_t*>(pd + k0);
-      const flt4_t a1 = va ? *reinterpret_cast<const flt4_t*>(pa + k0 + 16) : zero;
-      const flt4_t b1 = vb ? *reinterpret_cast<const flt4_t*>(pb + k0 + 16) : zero;
-      const flt4_t d1 = *reinterpret_cast<const flt4_t*>(pd + k0 + 16);
-#pragma unroll
-      for (int u = 0; u < 4; ++u) acc = Prec<float>::mfma(a0[u] * d0[u], b0[u] * d0[u], acc);
-#pragma unroll
-      for (int u = 0; u < 4; ++u) acc = Prec<float>::mfma(a1[u] * d1[u], b1[u] * d1[u], acc);
+  for (int q = wv; q < nitems; q += kWaves) {
+    int I = 0, rem = q;
+    while (rem >= I / 4 + 1) {
+      rem -= I / 4 + 1;
+      ++I;
     }
-    // lane holds column c0 + lrow, rows r0 + Prec<float>::row(lk, g)
-    const double sj = cb < m ? L.sc[cb] : 0.0;
+    const int J0 = 4 * rem;
+    const int nj = min(4, I + 1 - J0);
+    const int ra = 16 * I + lrow;
+    const bool va = ra < m;
+    const float* pa = L.Af + static_cast<long long>(va ? ra : 0) * ldr + 4 * lk;
+    const float* pd = L.sd + 4 * lk;
+    const float* pb[4];
+    bool vb[4];
 #pragma unroll
-    for (int g = 0; g < 4; ++g) {
-      const int i = r0 + Prec<float>::row(lk, g), j = cb;
-      if (i < m && j < m && i >= j) {
-        const double si = L.sc[i];
-        M[packed_index(i, j, m)] = static_cast<float>(si * sj * (static_cast<double>(acc[g]) + (i == j ? reg : 0.0)));
+    for (int c = 0; c < 4; ++c) {
+      const int cb = 16 * (J0 + c) + lrow;
+      vb[c] = c < nj && cb < m;
+      pb[c] = L.Af + static_cast<long long>(vb[c] ? cb : 0) * ldr + 4 * lk;
+    }
+    flt4_t acc[4] = {zero, zero, zero, zero};
+    for (int k0 = 0; k0 < ldr; k0 += 32) {  // ldr is a multiple of 32
+      const flt4_t d0 = *reinterpret_cast<const flt4_t*>(pd + k0);
+      const flt4_t d1 = *reinterpret_cast<const flt4_t*>(pd + k0 + 16);
+      flt4_t a0 = va ? *reinterpret_cast<const flt4_t*>(pa + k0) : zero;
+      flt4_t a1 = va ? *reinterpret_cast<const flt4_t*>(pa + k0 + 16) : zero;
+      flt4_t b0[4], b1[4];
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        b0[c] = vb[c] ? *reinterpret_cast<const flt4_t*>(pb[c] + k0) : zero;
+        b1[c] = vb[c] ? *reinterpret_cast<const flt4_t*>(pb[c] + k0 + 16) : zero;
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        a0[u] *= d0[u];
+        a1[u] *= d1[u];
+      }
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        if (c < nj) {
+#pragma unroll
+          for (int u = 0; u < 4; ++u) acc[c] = Prec<float>::mfma(a0[u], b0[c][u] * d0[u], acc[c]);
+#pragma unroll
+          for (int u = 0; u < 4; ++u) acc[c] = Prec<float>::mfma(a1[u], b1[c][u] * d1[u], acc[c]);
+        }
+      }
+    }
+    // lane holds column 16 (J0 + c) + lrow, rows 16 I + Prec<float>::row(lk, g)
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      const int j = 16 * (J0 + c) + lrow;
+      if (c >= nj || j >= m) continue;
+      const double sj = L.sc[j];
+#pragma unroll
+      for (int g = 0; g < 4; ++g) {
+        const int i = 16 * I + Prec<float>::row(lk, g);
+        if (i < m && i >= j) {
+          const double si = L.sc[i];
+          M[packed_index(i, j, m)] =
+              static_cast<float>(si * sj * (static_cast<double>(acc[c][g]) + (i == j ? reg : 0.0)));
+        }
       }
     }
   }
