@@ -23,9 +23,9 @@ per launch comes from rocprofv3 FETCH_SIZE / WRITE_SIZE passes recorded in
 profiles/traffic.json (null when absent).
 
 CPU baseline: the oracle's restatement of the reference CPU interior-point
-path (minimum-degree ordered sparse LDL^T, one thread) on two bounded
-samples of the same generator (m = 1024 and 2048), extrapolated to m = 8192
-with the exponent fitted between them.
+path (minimum-degree ordered sparse LDL^T, one thread): its phases timed at
+m = 4096 and scaled to m = 8192 by the per-phase ratios of a full-size run
+committed under profiles/ (python bench.py --cpu-phases 4096 8192 makes it).
 
 Separable QP: 20,002 variables, 20,000 second-difference rows, solved on
 the mixed and the plain FP64 path (synthetic target and the reference's own
@@ -242,24 +242,75 @@ def K_pf(a):
     return a.ctypes.data_as(C.POINTER(C.c_double))
 
 
-def cpu_baseline(sizes=(1024, 2048)):
+def cpu_phases(m, say=None):
+    """The oracle's restatement of the reference CPU interior-point path on
+    the random LP of size m (4m columns, 6 nonzeros a column, seed 26119),
+    phase by phase: the NormalEquations constructor (normal-matrix pattern,
+    greedy minimum-degree ordering, symbolic analysis; once per solve), one
+    numeric LDL^T factorization and one solve (twice per iteration), one
+    thread. Returns seconds per phase."""
     from oracle import oracle as O
     from sovereign_amd import problems as P
 
-    runs = []
-    for m in sizes:
-        lp = P.random_sparse_lp(m, 4 * m, 6, seed=P.BASE_SEED)
-        t0 = time.perf_counter()
-        r = O.solve_form(lp, "auto", feasibility_tol=1e-8, optimality_tol=1e-8)
-        runs.append((m, time.perf_counter() - t0, r["iterations"], r["status"]))
-    (m1, t1, _, _), (m2, t2, it2, st2) = runs[0], runs[-1]
-    expo = math.log(t2 / t1) / math.log(m2 / m1)
-    value = t2 * (8192 / m2) ** expo
+    lp = P.random_sparse_lp(m, 4 * m, 6, seed=P.BASE_SEED)
+    L = O.lib()
+    out = {"m": m}
+    t = time.perf_counter()
+    h = L.orc_ne_create(lp.m, lp.n, O.pi(O.i32(lp.col_ptr)), O.pi(O.i32(lp.row_idx)), O.pf(O.f64(lp.values)), 0)
+    out["create_s"] = time.perf_counter() - t
+    if say:
+        say(f"cpu phases m={m}: create (pattern, ordering, symbolic) {out['create_s']:.2f}s")
+    d = O.f64(lp.x0 / lp.s0)
+    t = time.perf_counter()
+    L.orc_ne_factor(h, O.pf(d))
+    out["factor_s"] = time.perf_counter() - t
+    out["factor_nnz"] = int(L.orc_ne_factor_nnz(h))
+    if say:
+        say(f"cpu phases m={m}: numeric factorization {out['factor_s']:.2f}s, nnz(L) {out['factor_nnz']}")
+    r = np.ones(lp.m)
+    t = time.perf_counter()
+    L.orc_ne_solve(h, O.pf(r))
+    out["solve_s"] = time.perf_counter() - t
+    L.orc_ne_free(h)
+    if say:
+        say(f"cpu phases m={m}: solve {out['solve_s']:.3f}s")
+    return out
+
+
+def load_cpu_scaling():
+    path = os.path.join(ROOT, "profiles", "cpu_oracle_ipm_phases.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as fh:
+        return json.load(fh)
+
+
+def cpu_baseline(iterations, sample_m=4096):
+    """The oracle's CPU path (one thread) on a bounded sample of the
+    workload: its phases timed live at m = 4096, scaled to m = 8192 by the
+    per-phase ratios of a full-size run on the same kind of host
+    (profiles/cpu_oracle_ipm_phases.json, both sizes timed in one run), then
+    one solve = the constructor + iterations x (factorization + 2 solves)."""
+    ph = cpu_phases(sample_m, say=log)
+    sc = load_cpu_scaling()
+    if sc is None:
+        return {"value": None, "unit": "s per IPM solve (m=8192)", "cores": 1, "kind": "port",
+                "sample": f"phases at m={sample_m} only (no full-size scaling profile committed)", "phases": ph}
+    r_create = sc["8192"]["create_s"] / sc[str(sample_m)]["create_s"]
+    r_factor = sc["8192"]["factor_s"] / sc[str(sample_m)]["factor_s"]
+    r_solve = sc["8192"]["solve_s"] / sc[str(sample_m)]["solve_s"]
+    create = ph["create_s"] * r_create
+    per_it = ph["factor_s"] * r_factor + 2 * ph["solve_s"] * r_solve
     return {
-        "value": value, "unit": "s per IPM solve (extrapolated to m=8192)", "cores": 1, "kind": "port",
-        "sample": (f"oracle restatement of the reference CPU IPM (AMD-ordered sparse LDL^T, one thread), full "
-                   f"solves of the same generator at m={m1} ({t1:.2f} s) and m={m2} ({t2:.2f} s, {it2} iterations, "
-                   f"status {st2}); time ~ m^{expo:.2f} fitted between them, extrapolated to m=8192"),
+        "value": create + iterations * per_it, "unit": "s per IPM solve (m=8192 LP, 1e-8 gap)", "cores": 1,
+        "kind": "port", "nproc": os.cpu_count(),
+        "sample": (f"oracle restatement of the reference CPU interior-point path (NormalEquations: greedy minimum-degree "
+                   f"ordering + up-looking LDL^T, one thread), phases timed live at m={sample_m} "
+                   f"(constructor {ph['create_s']:.1f} s, factorization {ph['factor_s']:.2f} s, solve "
+                   f"{ph['solve_s']:.3f} s), scaled to m=8192 by the ratios of a full-size run committed in "
+                   f"profiles/cpu_oracle_ipm_phases.json (constructor x{r_create:.2f}, factorization x{r_factor:.2f}); "
+                   f"one solve = constructor + {iterations} iterations x (factorization + 2 solves)"),
+        "phases_live": ph, "per_iteration_s": per_it, "constructor_s": create,
     }
 
 
@@ -451,7 +502,14 @@ def main():
     ap.add_argument("--m", type=int, default=8192)
     ap.add_argument("--n", type=int, default=32768)
     ap.add_argument("--copy-probe", action="store_true")
+    ap.add_argument("--cpu-phases", type=int, nargs="*", default=None,
+                    help="time the oracle's CPU phases at these sizes (the full-size scaling profile)")
     args = ap.parse_args()
+    if args.cpu_phases is not None:
+        rec = {str(m): cpu_phases(m, say=log) for m in (args.cpu_phases or [4096, 8192])}
+        rec["nproc"] = os.cpu_count()
+        print(json.dumps(rec), flush=True)
+        return
     if args.copy_probe:
         import torch
 
@@ -568,8 +626,8 @@ def main():
     if not args.no_qp:
         line["separable_qp"] = qp_leg(max(3, args.steps), not args.no_cpu)
     if not args.no_cpu:
-        log("cpu baseline (oracle, m=1024 and m=2048)")
-        line["cpu_baseline"] = cpu_baseline()
+        log("cpu baseline (oracle phases at m=4096, scaled to m=8192)")
+        line["cpu_baseline"] = cpu_baseline(line["iterations"][main_path])
     print(json.dumps(line), flush=True)
 
 
