@@ -73,6 +73,84 @@ int sov_qp_liswet_solve(int32_t n, const double* target, const sov_qp_options* o
 int sov_qp_liswet_polish(int32_t n, const double* target, const double* s, const double* z, sov_qp_result* res,
                          double* x_out, double* lambda_out);
 
+/* ----------------------------------------------------------- generic QP --
+ * QpInteriorPointSolver::solve (solver/qp/include/sovereign/qp_interior_point.hpp:
+ * 24-31) for a convex QP that is not of the structured shape: the generic
+ * branch, solve_qp_ipm (solver/qp/src/qp_interior_point.cpp:2848-3116) on
+ * the form of build_qp_form (:118-247), first from the default starting
+ * point, then from Mehrotra's when that fails (:3166-3180).
+ *
+ * Model: minimize (or maximize) 1/2 sum_t q_t x_{q_row[t]} x_{q_col[t]} +
+ * obj' x + obj_constant, the quadratic terms as the reference stores them (a
+ * diagonal term enters Q_ii, an off-diagonal term adds q/2 to Q_ij and to
+ * Q_ji), rows of a CSC matrix with senses 'L' / 'G' / 'E', bounds (lb <=
+ * -1e29: free, ub >= 1e29: none).
+ *
+ * Method (GPU, FP64): the Newton system [Q + X^-1 S, -A^T; A, 0] of QpKkt
+ * (:398-656) in Schur-complement form: K = Q + X^-1 S + delta I by the
+ * library's blocked dense Cholesky on MFMA, S = A K^-1 A^T + delta I as the
+ * MFMA product V'V with V' = A L^-T (block triangular solve through the
+ * factor's diagonal-block inverses), then S by the same Cholesky. delta
+ * starts at 1e-9 and is raised 100-fold on a breakdown up to 1e-5 (the
+ * reference's quasi-definite regularisation, :386-388); each solve is
+ * refined against the unregularised system at most 3 times, a step kept
+ * only when it halves the residual (:469-505). Status codes are
+ * SolverStatus: 0 Optimal, 7 Error (factorization or iteration limit). */
+typedef struct {
+  int32_t n_vars;
+  int32_t n_rows;
+  const int32_t* col_ptr;
+  const int32_t* row_idx;
+  const double* values;
+  const double* obj;
+  double obj_constant;
+  int32_t maximize;
+  int32_t pad;
+  const double* lb;
+  const double* ub;
+  const char* row_sense;
+  const double* rhs;
+  int64_t q_nnz;
+  const int32_t* q_row;
+  const int32_t* q_col;
+  const double* q_val;
+} sov_qp_model;
+
+/* QpInteriorPointOptions (qp_interior_point.hpp:7-18). */
+typedef struct {
+  int32_t max_iterations;      /* 100 */
+  int32_t iterative_refinement; /* 1 */
+  double feasibility_tol;      /* 1e-8 */
+  double optimality_tol;       /* 1e-8 */
+  double fraction_to_boundary; /* 0.999 */
+  int32_t common_step;         /* 1: equal primal and dual steps when Q != 0 */
+  int32_t retry_mehrotra_start; /* 1 */
+} sov_qp_ipm_options;
+
+typedef struct {
+  int32_t status;     /* 0 Optimal, 7 Error */
+  int32_t iterations; /* both attempts when the second one ran */
+  double objective;   /* of the model, constant included */
+  double duality_gap;
+  double primal_residual;
+  double dual_residual;
+  int32_t factorizations;     /* K and S factorizations */
+  int32_t refinement_steps;   /* accepted refinement steps */
+  int32_t mehrotra_start;     /* 1 when the answer came from the second attempt */
+  int32_t n_form;             /* columns of the interior-point form (slacks included) */
+  int32_t m_form;             /* rows of the form (upper-bound rows included) */
+  int32_t pad;
+  double regularization;      /* delta at the end */
+  double seconds;
+} sov_qp_ipm_result;
+
+void sov_qp_ipm_default_options(sov_qp_ipm_options* opt);
+
+/* primal (n_vars), row_dual (the form's m_form rows: the model's rows, then
+ * one per finite upper bound) and var_dual (n_vars) may be NULL. */
+int sov_qp_ipm_solve(const sov_qp_model* model, const sov_qp_ipm_options* opt, sov_qp_ipm_result* res,
+                     double* primal, double* row_dual, double* var_dual);
+
 #ifdef __cplusplus
 }
 #endif

@@ -79,6 +79,8 @@ _SIGS = {
 }
 
 _REF_SIGS = {
+    "ref_qp_solve": (C.c_int, [C.c_int32, C.c_int32, _f64p, C.c_char_p, _f64p, _f64p, _f64p, _f64p, _f64p,
+                               C.c_int32, C.c_double, _f64p, _f64p]),
     "ref_lp_solve": _SIGS["orc_lp_solve"],
     "ref_dense_lu_solve": (C.c_int, [C.c_int32, _f64p, _f64p, C.c_int]),
     "ref_spmv_csc": _SIGS["orc_spmv_csc"],
@@ -347,3 +349,17 @@ def liswet_certify(target, x):
     d = dict(zip(keys, out.tolist()))
     d["ok"] = bool(d["ok"])
     return d, lam
+
+
+def ref_qp_solve(A, senses, rhs, lb, ub, c, Q, maximize=False, constant=0.0):
+    """The reference's own QpInteriorPointSolver (needs _ref) on a dense QP:
+    1/2 x'Qx + c'x + constant, rows A x (sense) rhs."""
+    A = np.ascontiguousarray(A, dtype=np.float64)
+    m, n = A.shape
+    Q = np.ascontiguousarray(Q, dtype=np.float64)
+    out = np.zeros(6)
+    x = np.zeros(n)
+    ref().ref_qp_solve(n, m, pf(A), "".join(senses).encode(), pf(f64(rhs)), pf(f64(lb)), pf(f64(ub)), pf(f64(c)),
+                       pf(Q), 1 if maximize else 0, float(constant), pf(out), pf(x))
+    return {"status": int(out[0]), "iterations": int(out[1]), "objective": out[2], "duality_gap": out[3],
+            "primal_residual": out[4], "dual_residual": out[5], "x": x}

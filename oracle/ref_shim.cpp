@@ -11,6 +11,7 @@
 //   DenseLU                      solver/numerical/include/sovereign/dense_lu.hpp:9-23
 //   spmv_csc_auto                solver/gpu/include/sovereign/gpu_spmv.hpp:22-31
 //   build_normal_eq_pattern/values, SparseLDLT, AMDOrdering (numerical/)
+//   QpInteriorPointSolver::solve solver/qp/include/sovereign/qp_interior_point.hpp:24-31
 
 #include "sov_oracle.h"
 
@@ -18,6 +19,7 @@
 #include "sovereign/dense_lu.hpp"
 #include "sovereign/gpu_spmv.hpp"
 #include "sovereign/interior_point.hpp"
+#include "sovereign/qp_interior_point.hpp"
 #include "sovereign/sparse_ldlt.hpp"
 #include "sovereign/sparse_matrix.hpp"
 #include "sovereign/sparse_symmetric.hpp"
@@ -179,6 +181,56 @@ int ref_amd_order(int32_t nn, const int32_t* pcp, const int32_t* pri, int64_t li
     iperm[k] = ipv[k];
   }
   return ok ? 1 : 0;
+}
+
+// The reference's QpInteriorPointSolver on a dense model: A (m x n,
+// row-major), senses 'L'/'G'/'E', Q (n x n, row-major, symmetric) of
+// 1/2 x'Qx, stored as one term per nonzero entry (a term (i, j) adds q/2 to
+// Q_ij and Q_ji, so each symmetric pair is stored twice with its own value).
+// out6: status, iterations, objective, duality gap, primal and dual residual.
+int ref_qp_solve(int32_t n, int32_t m, const double* A, const char* senses, const double* rhs,
+                 const double* lb, const double* ub, const double* c, const double* Q, int32_t maximize,
+                 double constant, double* out6, double* primal) {
+  OptimizationModel model;
+  model.problem_type = ProblemType::QP;
+  model.sense = maximize ? Sense::Maximize : Sense::Minimize;
+  model.objective.constant = constant;
+  for (int j = 0; j < n; ++j) {
+    Variable v;
+    v.name = var_name(j);
+    v.lower_bound = lb[j];
+    v.upper_bound = ub[j];
+    model.variables.push_back(v);
+    if (c[j] != 0.0) model.objective.linear[v.name] = c[j];
+  }
+  for (int i = 0; i < n; ++i)
+    for (int j = 0; j < n; ++j) {
+      const double q = Q[static_cast<std::size_t>(i) * n + j];
+      if (q != 0.0) model.objective.quadratic[var_name(i)][var_name(j)] = q;
+    }
+  for (int r = 0; r < m; ++r) {
+    Constraint cons;
+    cons.name = "r" + std::to_string(r);
+    cons.sense = senses[r] == 'G' ? ConstraintSense::Ge : senses[r] == 'E' ? ConstraintSense::Eq : ConstraintSense::Le;
+    cons.rhs = rhs[r];
+    for (int j = 0; j < n; ++j) {
+      const double a = A[static_cast<std::size_t>(r) * n + j];
+      if (a != 0.0) cons.linear[var_name(j)] = a;
+    }
+    model.constraints.push_back(cons);
+  }
+  const SolverResult r = QpInteriorPointSolver().solve(model);
+  out6[0] = static_cast<double>(status_code(r.status));
+  out6[1] = static_cast<double>(r.iterations);
+  out6[2] = r.has_objective_value ? r.objective_value : 0.0;
+  out6[3] = r.duality_gap;
+  out6[4] = r.primal_residual;
+  out6[5] = r.dual_residual;
+  for (int j = 0; j < n; ++j) {
+    auto it = r.primal.find(var_name(j));
+    primal[j] = it == r.primal.end() ? 0.0 : it->second;
+  }
+  return 0;
 }
 
 }  // extern "C"

@@ -164,6 +164,35 @@ class QpResult(C.Structure):
         return {name: getattr(self, name) for name, _ in self._fields_ if name != "pad"}
 
 
+class QpModelC(C.Structure):
+    _fields_ = [
+        ("n_vars", C.c_int32), ("n_rows", C.c_int32), ("col_ptr", _i32p), ("row_idx", _i32p), ("values", _f64p),
+        ("obj", _f64p), ("obj_constant", C.c_double), ("maximize", C.c_int32), ("pad", C.c_int32),
+        ("lb", _f64p), ("ub", _f64p), ("row_sense", C.c_char_p), ("rhs", _f64p), ("q_nnz", C.c_int64),
+        ("q_row", _i32p), ("q_col", _i32p), ("q_val", _f64p),
+    ]
+
+
+class QpIpmOptions(C.Structure):
+    _fields_ = [
+        ("max_iterations", C.c_int32), ("iterative_refinement", C.c_int32), ("feasibility_tol", C.c_double),
+        ("optimality_tol", C.c_double), ("fraction_to_boundary", C.c_double), ("common_step", C.c_int32),
+        ("retry_mehrotra_start", C.c_int32),
+    ]
+
+
+class QpIpmResult(C.Structure):
+    _fields_ = [
+        ("status", C.c_int32), ("iterations", C.c_int32), ("objective", C.c_double), ("duality_gap", C.c_double),
+        ("primal_residual", C.c_double), ("dual_residual", C.c_double), ("factorizations", C.c_int32),
+        ("refinement_steps", C.c_int32), ("mehrotra_start", C.c_int32), ("n_form", C.c_int32),
+        ("m_form", C.c_int32), ("pad", C.c_int32), ("regularization", C.c_double), ("seconds", C.c_double),
+    ]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_ if name != "pad"}
+
+
 _f32p = C.POINTER(C.c_float)
 
 SIGNATURES = {
@@ -223,6 +252,9 @@ SIGNATURES = {
     "sov_qp_default_options": (None, [C.POINTER(QpOptions)]),
     "sov_qp_liswet_solve": (C.c_int, [C.c_int32, _f64p, C.POINTER(QpOptions), C.POINTER(QpResult), _f64p, _f64p]),
     "sov_qp_liswet_polish": (C.c_int, [C.c_int32, _f64p, _f64p, _f64p, C.POINTER(QpResult), _f64p, _f64p]),
+    "sov_qp_ipm_default_options": (None, [C.POINTER(QpIpmOptions)]),
+    "sov_qp_ipm_solve": (C.c_int, [C.POINTER(QpModelC), C.POINTER(QpIpmOptions), C.POINTER(QpIpmResult), _f64p,
+                                   _f64p, _f64p]),
 }
 
 _lib = None

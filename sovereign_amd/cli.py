@@ -129,12 +129,16 @@ def solve_qp(model: IO.Model) -> IO.Result:
     rows = [{idx[n]: v for n, v in lin_r.items()} for lin_r in model.con_linear]
     qm = qp.QpModel(n_vars=model.n_vars, linear=lin, quadratic=quad, rows=rows, senses=list(model.con_senses),
                     rhs=np.array(model.con_rhs), lb=np.array(model.lb), ub=np.array(model.ub),
-                    minimize=model.sense == "minimize")
+                    minimize=model.sense == "minimize", constant=model.obj_constant)
     r = qp.QpInteriorPointSolver().solve(qm)
     out = IO.Result(status=IO.STATUS.get(r["status"], "ERROR"), message=r.get("message", "") or "",
                     iterations=int(r.get("iterations", 0)))
     if "x" in r and out.status in ("OPTIMAL", "FEASIBLE"):
         out.primal = {n: float(v) for n, v in zip(model.var_names, r["x"])}
+        if "var_dual" in r:
+            out.dual = {n: float(v) for n, v in zip(model.var_names, r["var_dual"])}
+            for i, v in enumerate(r.get("row_dual", [])):
+                out.dual[f"row_{i}"] = float(v)
     for k in ("duality_gap", "primal_residual", "dual_residual"):
         if k in r:
             setattr(out, k, float(r[k]))

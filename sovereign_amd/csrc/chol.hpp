@@ -42,6 +42,9 @@ class DenseCholesky {
   // The launch sequence is replayed as a graph per (a, x, m).
   int solve(const T* a, int m, double* x, double* work, hipStream_t s);
   FactorFlags* flags() const { return flags_; }
+  // Inverses of the factor's kSuper x kSuper diagonal blocks (block P at
+  // P * kSuper * kSuper, leading dimension kSuper, zero above the diagonal).
+  const T* super_inverses() const { return sup_inv_; }
 
  private:
   int build_super_inverses(const T* a, int m, hipStream_t s);

@@ -10,13 +10,16 @@ Mirrors the reference's QP entry point for this shape:
   the structured dispatch              qp_interior_point.cpp:3153-3161
   certify_liswet                       qp_interior_point.cpp:1410-1480
 
-A model that is not of this shape gets status NotImplemented: the generic
-sparse QP interior-point path of the reference is not part of this library.
+Any other convex QP goes to the generic interior point (sov_qp_ipm_solve):
+solve_qp_ipm (qp_interior_point.cpp:2848-3116) on build_qp_form's form
+(:118-247), from the default starting point and then Mehrotra's
+(:3166-3180), its Newton system in Schur-complement form on the GPU.
 Everything runs through libsovereign_hip.so; there is no CPU fallback.
 """
 from __future__ import annotations
 
 import ctypes as C
+import os
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -44,6 +47,7 @@ class QpModel:
     lb: np.ndarray = field(default=None)
     ub: np.ndarray = field(default=None)
     minimize: bool = True
+    constant: float = 0.0
 
     def __post_init__(self):
         self.linear = np.asarray(self.linear, dtype=np.float64)
@@ -149,21 +153,109 @@ def polish_liswet(target, s, z) -> dict:
     return out
 
 
-class QpInteriorPointSolver:
-    """QpInteriorPointSolver (qp_interior_point.hpp:24-31) for the structured
-    shape. solve(QpModel) -> dict (status 0 Optimal with a certificate, 6
-    NumericalError when the certificate fails, 8 NotImplemented for any other
-    shape)."""
+@dataclass
+class QpIpmOptions:
+    """QpInteriorPointOptions (qp_interior_point.hpp:7-18) of the generic path."""
+    max_iterations: int = 100
+    feasibility_tol: float = 1e-8
+    optimality_tol: float = 1e-8
+    fraction_to_boundary: float = 0.999
+    iterative_refinement: bool = True
+    common_step: bool = True
+    retry_mehrotra_start: bool = True
 
-    def __init__(self, options: QpInteriorPointOptions | None = None):
+    def to_c(self) -> K.QpIpmOptions:
+        o = K.QpIpmOptions()
+        o.max_iterations = self.max_iterations
+        o.feasibility_tol = self.feasibility_tol
+        o.optimality_tol = self.optimality_tol
+        o.fraction_to_boundary = self.fraction_to_boundary
+        o.iterative_refinement = int(self.iterative_refinement)
+        o.common_step = int(self.common_step)
+        o.retry_mehrotra_start = int(self.retry_mehrotra_start)
+        return o
+
+
+def solve_generic(model: QpModel, options: QpIpmOptions | None = None) -> dict:
+    """The generic QP interior point on the GPU. model.quadratic holds terms
+    q_ij of 1/2 sum q_ij x_i x_j as the reference stores them: a diagonal
+    term is Q_ii, an off-diagonal one adds q_ij / 2 to Q_ij and Q_ji, so the
+    entries of a symmetric Q, both (i, j) and (j, i), give that Q."""
+    lib = K.load()
+    _require_device()
+    n, m = model.n_vars, len(model.rows)
+    cols = [[] for _ in range(n)]
+    for r, row in enumerate(model.rows):
+        for j, v in row.items():
+            cols[int(j)].append((r, float(v)))
+    ptr = np.zeros(n + 1, dtype=np.int32)
+    ri, vals = [], []
+    for j in range(n):
+        for r, v in sorted(cols[j]):
+            ri.append(r)
+            vals.append(v)
+        ptr[j + 1] = len(ri)
+    ri = np.array(ri, dtype=np.int32)
+    vals = np.array(vals, dtype=np.float64)
+    qi = np.array([i for (i, j) in model.quadratic], dtype=np.int32)
+    qj = np.array([j for (i, j) in model.quadratic], dtype=np.int32)
+    qv = np.array(list(model.quadratic.values()), dtype=np.float64)
+    sense = "".join({">=": "G", "<=": "L", "=": "E"}[s] for s in model.senses).encode()
+    lb, ub, c, rhs = K.f64(model.lb), K.f64(model.ub), K.f64(model.linear), K.f64(model.rhs)
+    st = K.QpModelC()
+    st.n_vars, st.n_rows = n, m
+    st.col_ptr, st.row_idx, st.values = K.pi(ptr), K.pi(ri), K.pf(vals)
+    st.obj, st.obj_constant, st.maximize = K.pf(c), float(getattr(model, "constant", 0.0)), 0 if model.minimize else 1
+    st.lb, st.ub, st.row_sense, st.rhs = K.pf(lb), K.pf(ub), sense, K.pf(rhs)
+    st.q_nnz, st.q_row, st.q_col, st.q_val = len(qv), K.pi(qi), K.pi(qj), K.pf(qv)
+    res = K.QpIpmResult()
+    opt = (options or QpIpmOptions()).to_c()
+    x, y, z = np.zeros(n), np.zeros(max(m + n, 1)), np.zeros(n)
+    K.check(lib.sov_qp_ipm_solve(C.byref(st), C.byref(opt), C.byref(res), K.pf(x), K.pf(y), K.pf(z)),
+            "sov_qp_ipm_solve")
+    out = res.as_dict()
+    out["status_name"] = STATUS_NAMES.get(res.status, str(res.status))
+    out.update(x=x, row_dual=y[:res.m_form], var_dual=z)
+    return out
+
+
+class QpInteriorPointSolver:
+    """QpInteriorPointSolver (qp_interior_point.hpp:24-31): the structured
+    shape goes to the LISWET kernel (status 0 Optimal with a certificate, 6
+    NumericalError when the certificate fails), any other convex QP to the
+    generic interior point (0 Optimal, 7 Error)."""
+
+    def __init__(self, options: QpInteriorPointOptions | None = None, generic: QpIpmOptions | None = None):
         self.options = options or QpInteriorPointOptions()
+        self.generic = generic or QpIpmOptions()
 
     def solve(self, model: QpModel) -> dict:
         target = detect_liswet(model)
         if target is None:
-            return {"status": 8, "status_name": STATUS_NAMES[8],
-                    "message": "only the structured separable QP (second-difference rows) runs on the GPU"}
+            return solve_generic(model, generic_options_from_env(self.generic))
         return solve_liswet(target, self.options)
+
+
+def generic_options_from_env(base: QpIpmOptions) -> QpIpmOptions:
+    """The environment switches QpInteriorPointSolver::solve reads
+    (qp_interior_point.cpp:3139-3151): SOVEREIGN_QP_MAX_ITERATIONS (at least
+    1; malformed text keeps the default) and SOVEREIGN_QP_DISABLE, a comma
+    list that may name refinement, common_step and retry."""
+    o = QpIpmOptions(**vars(base))
+    text = os.environ.get("SOVEREIGN_QP_MAX_ITERATIONS")
+    if text is not None:
+        try:
+            o.max_iterations = max(1, int(text))
+        except ValueError:
+            pass
+    off = "," + os.environ.get("SOVEREIGN_QP_DISABLE", "") + ","
+    if ",refinement," in off:
+        o.iterative_refinement = False
+    if ",common_step," in off:
+        o.common_step = False
+    if ",retry," in off:
+        o.retry_mehrotra_start = False
+    return o
 
 
 def liswet_target(n: int = 20002, seed: int = 20260501, noise_sd: float = 0.01) -> np.ndarray:

@@ -112,6 +112,18 @@ def test_solver_interface():
     out = qp.QpInteriorPointSolver().solve(qp.liswet_model(t))
     assert out["status"] == 0 and out["status_name"] == "Optimal"
     assert np.max(np.abs(out["x"] - FIX["x_1000"])) <= 1e-9
+    # off the structured shape (bounded columns): the generic interior point,
+    # checked against the oracle's restatement of solve_qp_ipm
+    from oracle import qp_oracle as QO
+
+    t = FIX["target_200"][:60]
     m = qp.liswet_model(t)
     m.lb = np.zeros(len(t))
-    assert qp.QpInteriorPointSolver().solve(m)["status"] == 8
+    out = qp.QpInteriorPointSolver().solve(m)
+    k = len(t) - 2
+    A = np.zeros((k, len(t)))
+    A[np.arange(k), np.arange(k)], A[np.arange(k), np.arange(k) + 1], A[np.arange(k), np.arange(k) + 2] = 1, -2, 1
+    ref = QO.qp_solve(A, ["G"] * k, np.zeros(k), m.lb, np.full(len(t), 1e30), -t, np.eye(len(t)))
+    assert out["status"] == ref["status"] == 0
+    assert out["objective"] == pytest.approx(ref["objective"], rel=1e-8, abs=1e-10)
+    assert np.all(out["x"] >= -1e-9)
