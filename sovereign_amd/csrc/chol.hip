@@ -21,6 +21,7 @@
 #include <algorithm>
 #include <atomic>
 #include <cstdlib>
+#include <utility>
 #include <vector>
 
 namespace sov {
@@ -79,18 +80,14 @@ __device__ __forceinline__ float read_lane<float>(float v, int lane) {
 
 // Each lane receives v from lane `src` (its own choice): ds_bpermute, which
 // goes through the LDS crossbar without touching LDS memory.
-template <typename T>
-__device__ __forceinline__ T bcast_from(T v, int src);
-template <>
-__device__ __forceinline__ double bcast_from<double>(double v, int src) {
+__device__ __forceinline__ float lane_pull(float v, int src) {
+  return __int_as_float(__builtin_amdgcn_ds_bpermute(4 * src, __float_as_int(v)));
+}
+__device__ __forceinline__ double lane_pull(double v, int src) {
   const long long b = __double_as_longlong(v);
   const int lo = __builtin_amdgcn_ds_bpermute(4 * src, static_cast<int>(b));
   const int hi = __builtin_amdgcn_ds_bpermute(4 * src, static_cast<int>(b >> 32));
   return __longlong_as_double((static_cast<long long>(hi) << 32) | static_cast<unsigned int>(lo));
-}
-template <>
-__device__ __forceinline__ float bcast_from<float>(float v, int src) {
-  return __int_as_float(__builtin_amdgcn_ds_bpermute(4 * src, __float_as_int(v)));
 }
 
 constexpr int kLeaf = 16;             // leaf width of the diagonal sub-block factorization
@@ -123,126 +120,191 @@ struct LeafMath<float> {
   }
 };
 
-// Cholesky factor L and inverse X = L^-1 of the h x h (h <= 16) block at D (column-major, leading dimension kPanelLd) by one
-// wavefront, in registers. The 64 lanes form an 8 x 8 grid (p = lane & 7,
-// q = lane >> 3); lane (p, q) holds the elements (p + 8r, q + 8s), r, s < 2,
-// of the block (f) and of X (x). Step j scales column j of L below the
-// diagonal and zeroes it on and above (the pivot goes to dg of the lane
-// holding (j, j)), scales row j of X, then applies the rank-1 updates
-// f -= L(:, j) L(:, j)^T and x -= L(:, j) X(j, :); their factors are fetched
-// from the lanes that own them, and as column j is zero on and above the
-// diagonal, the updates are unconditional. The pivot of step j + 1 is formed
-// as soon as column j is scaled, a(j+1, j+1) - L(j+1, j)^2 from two lane
-// reads, so that its reciprocal square root overlaps the rest of step j.
-// Pivot arithmetic is in T. Writes L (lower), X (strict upper: X(i, k) at
-// D[i * kPanelLd + k]) and 1 / L(j, j) to xd[j]; counts dropped pivots and
-// breakdowns into flags once at the end. thr (lane j): drop threshold of
-// column j. Pivot rule as described above the kernel.
-//
-// EXACT = false is the fast pass: it takes every pivot as is and, if any
-// pivot would have been dropped or broken down, returns false having written
-// nothing (the caller then runs the EXACT pass, which applies the rule).
-// Where no pivot needs the rule both passes compute the same values; the
-// fast pass saves the rule's per-step selects (a step is issue-bound: ~96
-// wave-wide instructions in FP64).
+template <typename T>
+struct LeafRcp;
+template <>
+struct LeafRcp<float> {
+  __device__ static __forceinline__ float rcp(float d) { return __builtin_amdgcn_rcpf(d); }
+};
+template <>
+struct LeafRcp<double> {
+  __device__ static __forceinline__ double rcp(double d) {
+    double r = __builtin_amdgcn_rcp(d);
+    r = fma(r, fma(-d, r, 1.0), r);
+    return fma(r, fma(-d, r, 1.0), r);
+  }
+};
+
+// 1 / d_j for the update of step j. EXACT = false takes the pivot as is and
+// clears `plain` when it would have needed the pivot rule; EXACT applies it
+// (drop: 1 / DropPivot^2, breakdown: 1).
 template <typename T, bool EXACT>
-__device__ __forceinline__ bool leaf_factor_pass(T* D, int h, T thr, T* xd, FactorFlags* flags, int lane) {
-  constexpr int NR = kLeaf / 8;
-  // The lane masks of the steps are formed from an opaque copy of the lane
-  // index, so that they are computed here and not hoisted out of the
-  // caller's sub-panel loop (where they would stay live and spill).
-  int ln = lane;
-  asm volatile("" : "+v"(ln));
-  const int p = ln & 7, q = ln >> 3;
-  T f[NR][NR], x[NR][NR], dg[NR];
-#pragma unroll
-  for (int r = 0; r < NR; ++r) {
-    dg[r] = T(0);
-#pragma unroll
-    for (int s2 = 0; s2 < NR; ++s2) {
-      const int i = p + 8 * r, k = q + 8 * s2;
-      f[r][s2] = (i < h && k <= i) ? D[k * kPanelLd + i] : T(0);
-      x[r][s2] = i == k ? T(1) : T(0);
-    }
+__device__ __forceinline__ T leaf_rinv(T d, int j, int h, T lim, bool& plain) {
+  if constexpr (EXACT) {
+    const bool active = j < h;
+    const bool finite = isfinite(d);
+    const bool drop = active & finite & (d <= lim);
+    const bool bad = active & !drop & (!finite | (d < T(1e-30)));
+    constexpr T kDrop = DropPivot<T>::value;
+    return drop ? T(1) / (kDrop * kDrop) : (bad ? T(1) : LeafRcp<T>::rcp(d));
+  } else {
+    plain = plain & (d > lim) & (d < T(INFINITY));
+    return LeafRcp<T>::rcp(d);
   }
-  T xdr = T(1);  // lane j: 1 / L(j, j)
-  int ndrop = 0;
-  bool anybad = false, plain = true;
-  T dnext = read_lane(f[0][0], 0);
-#pragma unroll
-  for (int j = 0; j < kLeaf; ++j) {
-    if (j < h) {  // (the guard also keeps the steps from being interleaved, which costs registers)
-      const int jp = j & 7, js = j >> 3;
-      const T dj = dnext;
-      const T rs = LeafMath<T>::rsqrt(dj);
-      T piv = dj * rs, rpiv = rs;
-      if constexpr (EXACT) {
-        const bool finite = isfinite(dj);
-        const bool drop = finite & (dj <= read_lane(thr, j));
-        const bool bad = !drop & (!finite | (dj < T(1e-30)));
-        ndrop += drop ? 1 : 0;
-        anybad = anybad || bad;
-        piv = drop ? DropPivot<T>::value : bad ? T(1) : piv;
-        rpiv = drop ? T(1) / DropPivot<T>::value : bad ? T(1) : rpiv;
-      } else {
-        // the negation of (dropped or broken down) in the rule above
-        plain = plain & (dj > read_lane(thr, j)) & (dj >= T(1e-30)) & isfinite(dj);
-      }
-      xdr = ln == j ? rpiv : xdr;
-      if (ln == 9 * jp) dg[js] = piv;
-      if (q == jp) {  // column j of L: a / pivot below the diagonal, zero on and above it
-#pragma unroll
-        for (int r = 0; r < NR; ++r) f[r][js] = p + 8 * r > j ? f[r][js] * rpiv : T(0);
-      }
-      if (j + 1 < h) {
-        const int j1 = j + 1, j1p = j1 & 7, j1s = j1 >> 3;
-        const T l = read_lane(f[j1s][js], j1p + 8 * jp);  // L(j+1, j)
-        const T a = read_lane(f[j1s][j1s], 9 * j1p);      // a(j+1, j+1) before this step's update
-        dnext = a - l * l;
-      }
-      if (p == jp) {  // row j of X
-#pragma unroll
-        for (int s2 = 0; s2 < NR; ++s2) x[js][s2] *= rpiv;
-      }
-      T lr[NR], lc[NR], xr[NR];
-#pragma unroll
-      for (int r = 0; r < NR; ++r) lr[r] = bcast_from(f[r][js], p + 8 * jp);  // L(p + 8r, j)
-#pragma unroll
-      for (int s2 = 0; s2 < NR; ++s2) {
-        lc[s2] = bcast_from(f[s2][js], q + 8 * jp);  // L(q + 8 s2, j)
-        xr[s2] = bcast_from(x[js][s2], jp + 8 * q);  // X(j, q + 8 s2)
-      }
-#pragma unroll
-      for (int r = 0; r < NR; ++r)
-#pragma unroll
-        for (int s2 = 0; s2 < NR; ++s2) {
-          f[r][s2] -= lr[r] * lc[s2];
-          x[r][s2] -= lr[r] * xr[s2];
-        }
-    }
-  }
-  if constexpr (!EXACT) {
-    if (!plain) return false;  // (uniform: every lane tests the same pivots)
-  }
-#pragma unroll
-  for (int r = 0; r < NR; ++r)
-#pragma unroll
-    for (int s2 = 0; s2 < NR; ++s2) {
-      const int i = p + 8 * r, k = q + 8 * s2;
-      if (i < h && k <= i) D[k * kPanelLd + i] = i == k ? dg[r] : f[r][s2];  // L(i, k)
-      if (i < h && k < i) D[i * kPanelLd + k] = x[r][s2];                    // X(i, k), above the diagonal
-    }
-  if (ln < h) xd[ln] = xdr;
-  if (EXACT && ln == 0) {
-    if (ndrop) atomicAdd(&flags->replaced, ndrop);
-    if (anybad) atomicOr(&flags->status, 1);
-  }
-  return true;
 }
 
+// x -= f(lane J) nsx' and f -= f(lane J) nt' for slots of the leaf, each one
+// instruction: the broadcast of lane J's f along its 16-lane row is the DPP
+// operand of the fused multiply-add (v_fmac_*_dpp row_newbcast:J), and
+// both read f before the second writes it. A DPP read of a register written
+// by one of the two instructions just before it needs two wait states, so
+// each statement starts and ends with s_nop 1 (the compiler's code around it
+// cannot see what it reads and writes).
+#define SOV_FMAC_PAIR(OP, F, X) \
+  OP " " X ", " F ", %[nsx] row_newbcast:%[j] row_mask:0xf bank_mask:0xf\n\t" OP " " F ", " F \
+     ", %[nt] row_newbcast:%[j] row_mask:0xf bank_mask:0xf\n\t"
+template <int J, typename T>
+__device__ __forceinline__ void leaf_fmac1(T& f0, T& x0, T nt, T nsx) {
+  if constexpr (sizeof(T) == 4)
+    asm volatile("s_nop 1\n\t" SOV_FMAC_PAIR("v_fmac_f32_dpp", "%[f0]", "%[x0]") "s_nop 1"
+                 : [f0] "+v"(f0), [x0] "+v"(x0)
+                 : [nt] "v"(nt), [nsx] "v"(nsx), [j] "n"(J));
+  else
+    asm volatile("s_nop 1\n\t" SOV_FMAC_PAIR("v_fmac_f64_dpp", "%[f0]", "%[x0]") "s_nop 1"
+                 : [f0] "+v"(f0), [x0] "+v"(x0)
+                 : [nt] "v"(nt), [nsx] "v"(nsx), [j] "n"(J));
+}
+template <int J, typename T>
+__device__ __forceinline__ void leaf_fmac3(T& f0, T& x0, T& f1, T& x1, T& f2, T& x2, T nt, T nsx) {
+  if constexpr (sizeof(T) == 4)
+    asm volatile("s_nop 1\n\t" SOV_FMAC_PAIR("v_fmac_f32_dpp", "%[f0]", "%[x0]")
+                     SOV_FMAC_PAIR("v_fmac_f32_dpp", "%[f1]", "%[x1]")
+                         SOV_FMAC_PAIR("v_fmac_f32_dpp", "%[f2]", "%[x2]") "s_nop 1"
+                 : [f0] "+v"(f0), [x0] "+v"(x0), [f1] "+v"(f1), [x1] "+v"(x1), [f2] "+v"(f2), [x2] "+v"(x2)
+                 : [nt] "v"(nt), [nsx] "v"(nsx), [j] "n"(J));
+  else
+    asm volatile("s_nop 1\n\t" SOV_FMAC_PAIR("v_fmac_f64_dpp", "%[f0]", "%[x0]")
+                     SOV_FMAC_PAIR("v_fmac_f64_dpp", "%[f1]", "%[x1]")
+                         SOV_FMAC_PAIR("v_fmac_f64_dpp", "%[f2]", "%[x2]") "s_nop 1"
+                 : [f0] "+v"(f0), [x0] "+v"(x0), [f1] "+v"(f1), [x1] "+v"(x1), [f2] "+v"(f2), [x2] "+v"(x2)
+                 : [nt] "v"(nt), [nsx] "v"(nsx), [j] "n"(J));
+}
+#undef SOV_FMAC_PAIR
+
+// Step J of the leaf (see leaf_factor). On entry nt = -f(J, i) / d_J (0 for
+// i <= J) and nsx = -Xu(J, i) / d_J, already on every row group, and d =
+// d_J. The step zeroes row J right of the diagonal (spent: no later step
+// may read it), updates the slot holding row J + 1 first and starts the
+// next step's chain from it (row J + 1 pulled to all groups, the pivot
+// d_{J+1} read from its lane, its reciprocal), then updates the other three
+// slots, which hides that chain, and finally forms the next multipliers.
+template <typename T, bool EXACT, int J>
+__device__ __forceinline__ void leaf_slot_step(T (&f)[4], T (&x)[4], int g, int i, int h, T lim, bool& plain, T& nt,
+                                               T& nsx, T& d, T& myd) {
+  constexpr int G = J >> 2, S = J & 3;
+  myd = i == J ? d : myd;
+  if constexpr (J + 1 < kLeaf) {
+    constexpr int S1 = (J + 1) & 3, G1 = (J + 1) >> 2;
+    f[S] = (g == G && i >= J) ? T(0) : f[S];
+    leaf_fmac1<J>(f[S1], x[S1], nt, nsx);
+    const T rf = lane_pull(f[S1], 16 * G1 + i), rx = lane_pull(x[S1], 16 * G1 + i);
+    d = read_lane(f[S1], 16 * G1 + J + 1);
+    const T rinv = leaf_rinv<T, EXACT>(d, J + 1, h, read_lane(lim, J + 1), plain);
+    __builtin_amdgcn_sched_barrier(0);
+    leaf_fmac3<J>(f[(J + 2) & 3], x[(J + 2) & 3], f[(J + 3) & 3], x[(J + 3) & 3], f[S], x[S], nt, nsx);
+    nt = i > J + 1 ? -(rf * rinv) : T(0);
+    nsx = -(rx * rinv);
+    asm volatile("" : "+v"(nt), "+v"(nsx));
+    __builtin_amdgcn_sched_barrier(0);
+  }
+}
+
+template <typename T, bool EXACT, int... J>
+__device__ __forceinline__ void leaf_slot_steps(T (&f)[4], T (&x)[4], int g, int i, int h, T lim, bool& plain,
+                                                T& myd, std::integer_sequence<int, J...>) {
+  T d = read_lane(f[0], 0);
+  const T rinv = leaf_rinv<T, EXACT>(d, 0, h, read_lane(lim, 0), plain);
+  T nt = i > 0 ? -(lane_pull(f[0], i) * rinv) : T(0);
+  T nsx = -(lane_pull(x[0], i) * rinv);
+  (leaf_slot_step<T, EXACT, J>(f, x, g, i, h, lim, plain, nt, nsx, d, myd), ...);
+}
+
+// Cholesky factor L and inverse X = L^-1 of the h x h (h <= 16) block at D
+// (column-major, leading dimension kPanelLd) by one wavefront. Lane
+// (g, i) = (lane >> 4, lane & 15) holds rows 4g .. 4g + 3 (slots) of column i
+// of the symmetric block f and of the unit-lower inverse Xu, in registers.
+// Step j: every lane updates all its slots,
+//   f(a, i) -= f(a, j) f(j, i) / d_j   (i > j),
+//   Xu(a, i) -= f(a, j) Xu(j, i) / d_j,
+// f(a, j) coming from lane j of the lane's own row group (DPP) and
+// f(j, i), Xu(j, i) from row group j >> 2 (ds_bpermute); row j of f right of
+// the diagonal is zeroed first, which makes the updates of rows <= j exact
+// no-ops, so no step needs a row mask. A step past the block (j >= h) meets
+// an identity pad and changes nothing. This is Gauss-Jordan on the
+// unit-lower factor of f = L_u D L_u^T; at the end L(a, i) = f(a, i) /
+// sqrt(d_i) and X(a, i) = Xu(a, i) / sqrt(d_a). The fast pass takes every
+// pivot as is; if one needed the pivot rule (as described above the kernel)
+// the exact pass reruns the block with it. Writes L lower with its
+// diagonal, X strictly above the diagonal (X(i, k) at D[i * kPanelLd + k])
+// and 1 / L(j, j) to xd[j]; counts drops and breakdowns into flags once.
+// thr: lane j (< 16) holds the drop threshold of column j.
 template <typename T>
 __device__ __forceinline__ void leaf_factor(T* D, int h, T thr, T* xd, FactorFlags* flags, int lane) {
-  if (!leaf_factor_pass<T, false>(D, h, thr, xd, flags, lane)) leaf_factor_pass<T, true>(D, h, thr, xd, flags, lane);
+  const int g = lane >> 4, i = lane & 15;
+  T f[4], x[4];
+  auto load = [&]() {
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+      const int a = 4 * g + s;
+      const bool in = i < h && a < h;
+      const int off = in ? (a >= i ? i * kPanelLd + a : a * kPanelLd + i) : 0;
+      const T v = D[off];
+      f[s] = in ? v : (a == i ? T(1) : T(0));  // identity pad past the block
+      x[s] = a == i ? T(1) : T(0);
+    }
+  };
+  load();
+  bool plain = true;
+  T myd = T(1);
+  const T lim = thr > T(1e-30) ? thr : T(1e-30);
+  leaf_slot_steps<T, false>(f, x, g, i, h, lim, plain, myd, std::make_integer_sequence<int, kLeaf>{});
+  bool exact = false;
+  if (!plain) {  // uniform
+    load();
+    myd = T(1);
+    leaf_slot_steps<T, true>(f, x, g, i, h, thr, plain, myd, std::make_integer_sequence<int, kLeaf>{});
+    exact = true;
+  }
+  const T d = myd;
+  T rs = LeafMath<T>::rsqrt(d), piv = d * rs;
+  if (exact) {
+    // the threshold of column i lives in lane i of row group 0 only
+    const T thr_i = lane_pull(thr, i);
+    const bool active = i < h;
+    const bool finite = isfinite(d);
+    const bool drop = active & finite & (d <= thr_i);
+    const bool bad = active & !drop & (!finite | (d < T(1e-30)));
+    rs = drop ? T(1) / DropPivot<T>::value : (bad ? T(1) : rs);
+    piv = drop ? DropPivot<T>::value : (bad ? T(1) : piv);
+    const unsigned long long nd = __ballot(drop && lane < 16), nb = __ballot(bad && lane < 16);
+    if (lane == 0) {
+      if (nd) atomicAdd(&flags->replaced, __popcll(nd));
+      if (nb) atomicOr(&flags->status, 1);
+    }
+  }
+  // 1 / L(a, a) of each slot's row a = 4g + s, from lane a of the own group
+  T rsa[4];
+#pragma unroll
+  for (int s = 0; s < 4; ++s) rsa[s] = lane_pull(rs, 20 * g + s);
+  if (lane < h) xd[i] = rs;
+  if (i < h) {
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+      const int a = 4 * g + s;
+      if (a < h && a >= i) D[i * kPanelLd + a] = a == i ? piv : f[s] * rs;  // L(a, i)
+      if (a < h && a > i) D[a * kPanelLd + i] = x[s] * rsa[s];              // X(a, i) = Xu(a, i) / L(a, a)
+    }
+  }
 }
 
 // Phase A for sub-panel c (width sb) by one wavefront: the two 16 x 16
