@@ -57,7 +57,7 @@ typedef void (*sov_candidate_order_fn)(int32_t n_vars, const double* x, const do
 #define SOV_BRANCH_FULL_STRONG 3     /* pick_strong_branch (:726-793) */
 
 /* BranchAndBoundOptions (branch_and_bound.hpp:18-59) for the parts built
- * here; cuts, heuristics and presolve are not. */
+ * here; heuristics and model presolve are not. */
 typedef struct {
   int64_t max_nodes;                /* 100000 */
   int32_t strong_branch_candidates; /* FullStrong: k candidates per node (the reference's default is 4) */
@@ -71,6 +71,10 @@ typedef struct {
   int32_t reliability_threshold; /* 4: observations per direction before a pseudocost is trusted */
   int32_t max_strong_per_node;   /* 8: reliability branching's strong-branch budget per node */
   int32_t plunging;              /* 1: continue with the preferred child before the best-bound queue */
+  int32_t max_cut_rounds;        /* 5: cut rounds at the root (0: no cuts), as max_cut_rounds */
+  int32_t max_cuts_per_node;     /* 20: cuts added per round */
+  int32_t cmir_at_root;          /* 0: cMIR separation off, the reference's default (cmir_max_depth = -1) */
+  int32_t pad;
 } sov_bnb_options;
 
 typedef struct {
@@ -91,12 +95,27 @@ typedef struct {
   double node_lp_seconds;
   int32_t max_depth;
   int32_t node_lp_failures;    /* subtrees dropped because their LP did not solve */
+  int32_t cuts_added;          /* root cuts kept in the model (cover + Gomory + cMIR) */
+  int32_t cuts_cover;
+  int32_t cuts_gomory;
+  int32_t cuts_cmir;
+  int32_t cut_rounds;          /* root rounds that added cuts */
+  int32_t cut_rejections;      /* candidates the validity gate rejected */
+  double root_bound_before_cuts; /* root LP objective (model sense) before the first round; NaN without cuts */
+  int32_t root_rows;           /* rows of the model the tree searches (base rows + cuts) */
+  int32_t pad2;
 } sov_bnb_result;
 
 void sov_bnb_default_options(sov_bnb_options* opt);
 
-/* BranchAndBoundSolver::solve (branch_and_bound.cpp:906-1523) without cuts,
- * heuristics or node presolve: best-bound queue with plunging, the node LP
+/* BranchAndBoundSolver::solve (branch_and_bound.cpp:906-1523) without
+ * heuristics or node presolve, with cuts at the root only: up to
+ * max_cut_rounds rounds of apply_cuts (:648-694, :1214-1273 at depth 0:
+ * covers, Gomory rounding and, when asked, cMIR, each through the validity
+ * gate, at most max_cuts_per_node a round, the root LP re-solved after each)
+ * whose cuts every node inherits; the reference also separates one round
+ * at every deeper node, which is not done here. With the batched child
+ * solver the cuts stop at 272 rows in all. Then: best-bound queue with plunging, the node LP
  * solved when the node is popped (reusing a child LP that strong branching
  * already solved), pseudocost learning from node LPs (:1159-1170) and from
  * strong branching, integrality checked before bound pruning (:1317), and

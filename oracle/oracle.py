@@ -79,6 +79,11 @@ _SIGS = {
 }
 
 _REF_SIGS = {
+    "ref_separate_cuts": (C.c_int, [C.POINTER(OrcLpModel), C.POINTER(C.c_int8), _f64p, C.c_int32, C.c_double, C.c_int32,
+                                    C.c_int32, C.c_int32, C.c_int64, C.POINTER(C.c_int32), C.POINTER(C.c_int64),
+                                    C.POINTER(C.c_int32), C.POINTER(C.c_int32), _f64p, _f64p]),
+    "ref_cut_validity": (C.c_int, [C.POINTER(OrcLpModel), C.c_int32, C.POINTER(C.c_int32), _f64p, C.c_char, C.c_double,
+                                   C.c_int32, _f64p, C.c_double]),
     "ref_qp_solve": (C.c_int, [C.c_int32, C.c_int32, _f64p, C.c_char_p, _f64p, _f64p, _f64p, _f64p, _f64p,
                                C.c_int32, C.c_double, _f64p, _f64p]),
     "ref_lp_solve": _SIGS["orc_lp_solve"],
@@ -363,3 +368,39 @@ def ref_qp_solve(A, senses, rhs, lb, ub, c, Q, maximize=False, constant=0.0):
                        pf(Q), 1 if maximize else 0, float(constant), pf(out), pf(x))
     return {"status": int(out[0]), "iterations": int(out[1]), "objective": out[2], "duality_gap": out[3],
             "primal_residual": out[4], "dual_residual": out[5], "x": x}
+
+
+def ref_separate_cuts(model, kind, x, family, int_tol=1e-6, limit=8, base_rows=-1):
+    """The reference's generate_cover_cuts (family 1), generate_mir_cuts (2) or
+    generate_cmir_cuts (4) on `model` at x (needs _ref). kind: 0 continuous,
+    1 integer, 2 binary. Returns [(dict var -> coef, rhs)], all '<=' rows."""
+    st = model.fill_struct(OrcLpModel)
+    kind = np.ascontiguousarray(kind, dtype=np.int8)
+    x = f64(x)
+    cap_cuts, cap_nnz = 256, 1 << 20
+    n_cuts, nnz = C.c_int32(0), C.c_int64(0)
+    row_ptr = np.zeros(cap_cuts + 1, dtype=np.int32)
+    col = np.zeros(cap_nnz, dtype=np.int32)
+    val = np.zeros(cap_nnz)
+    rhs = np.zeros(cap_cuts)
+    rc = ref().ref_separate_cuts(C.byref(st), kind.ctypes.data_as(C.POINTER(C.c_int8)), pf(x), family, int_tol, limit,
+                                 base_rows, cap_cuts, cap_nnz, C.byref(n_cuts), C.byref(nnz), pi(row_ptr), pi(col),
+                                 pf(val), pf(rhs))
+    if rc != 0:
+        raise RuntimeError("ref_separate_cuts: output capacity exceeded")
+    out = []
+    for k in range(n_cuts.value):
+        a, b = row_ptr[k], row_ptr[k + 1]
+        out.append(({int(j): float(v) for j, v in zip(col[a:b], val[a:b])}, float(rhs[k])))
+    return out
+
+
+def ref_cut_validity(model, coefs, sense, rhs, points, tol=1e-6):
+    """check_cut_validity (needs _ref): 0 accepted, 1 a point is cut off, 2 the
+    bound box cannot satisfy the cut."""
+    st = model.fill_struct(OrcLpModel)
+    cols = np.array(list(coefs.keys()), dtype=np.int32)
+    vals = np.array(list(coefs.values()), dtype=np.float64)
+    pts = np.ascontiguousarray(points, dtype=np.float64).reshape(-1, model.n_vars) if len(points) else np.zeros((0, model.n_vars))
+    return int(ref().ref_cut_validity(C.byref(st), len(cols), pi(cols), pf(vals), sense.encode(), float(rhs), len(pts),
+                                      pf(pts), float(tol)))

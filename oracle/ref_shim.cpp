@@ -12,10 +12,12 @@
 //   spmv_csc_auto                solver/gpu/include/sovereign/gpu_spmv.hpp:22-31
 //   build_normal_eq_pattern/values, SparseLDLT, AMDOrdering (numerical/)
 //   QpInteriorPointSolver::solve solver/qp/include/sovereign/qp_interior_point.hpp:24-31
+//   generate_cover/mir/cmir_cuts, check_cut_validity  solver/cuts/include/sovereign/cuts.hpp:17-55
 
 #include "sov_oracle.h"
 
 #include "sovereign/amd_ordering.hpp"
+#include "sovereign/cuts.hpp"
 #include "sovereign/dense_lu.hpp"
 #include "sovereign/gpu_spmv.hpp"
 #include "sovereign/interior_point.hpp"
@@ -231,6 +233,63 @@ int ref_qp_solve(int32_t n, int32_t m, const double* A, const char* senses, cons
     primal[j] = it == r.primal.end() ? 0.0 : it->second;
   }
   return 0;
+}
+
+// The reference's cut generators on the model's rows at x. kind: 0
+// continuous, 1 integer, 2 binary. family 1 cover, 2 Gomory, 4 cMIR. Cuts
+// come back in CSR form over variable indices (capacities max_cuts / max_nnz;
+// the sizes needed are reported either way); returns -1 when they do not fit.
+int ref_separate_cuts(const orc_lp_model* m, const int8_t* kind, const double* x, int32_t family, double int_tol,
+                      int32_t limit, int32_t base_rows, int32_t max_cuts, int64_t max_nnz, int32_t* n_cuts,
+                      int64_t* nnz, int32_t* row_ptr, int32_t* col, double* val, double* rhs) {
+  OptimizationModel model = to_model(*m);
+  model.problem_type = ProblemType::MILP;
+  for (int j = 0; j < m->n_vars; ++j)
+    model.variables[static_cast<std::size_t>(j)].type =
+        kind[j] == 2 ? VariableType::Binary : kind[j] == 1 ? VariableType::Integer : VariableType::Continuous;
+  std::unordered_map<std::string, double> point;
+  for (int j = 0; j < m->n_vars; ++j) point[var_name(j)] = x[j];
+  std::vector<Cut> cuts;
+  if (family == 1) cuts = generate_cover_cuts(model, point, int_tol, limit);
+  else if (family == 2) cuts = generate_mir_cuts(model, point, int_tol, limit);
+  else
+    cuts = generate_cmir_cuts(model, point, int_tol, limit,
+                              base_rows < 0 ? static_cast<std::size_t>(-1) : static_cast<std::size_t>(base_rows));
+  int64_t total = 0;
+  for (const auto& c : cuts) total += static_cast<int64_t>(c.constraint.linear.size());
+  *n_cuts = static_cast<int32_t>(cuts.size());
+  *nnz = total;
+  if (*n_cuts > max_cuts || total > max_nnz) return -1;
+  int64_t p = 0;
+  row_ptr[0] = 0;
+  for (std::size_t k = 0; k < cuts.size(); ++k) {
+    for (const auto& kv : cuts[k].constraint.linear) {
+      col[p] = std::atoi(kv.first.c_str() + 1);  // "v<j>"
+      val[p] = kv.second;
+      ++p;
+    }
+    row_ptr[k + 1] = static_cast<int32_t>(p);
+    rhs[k] = cuts[k].constraint.rhs;
+  }
+  return 0;
+}
+
+// check_cut_validity on one cut: 0 accepted, 1 a reference point is cut
+// off, 2 the bound box cannot satisfy it (the reason string's kind).
+int ref_cut_validity(const orc_lp_model* m, int32_t nnz, const int32_t* col, const double* val, char sense, double rhs,
+                     int32_t n_points, const double* points, double tol) {
+  OptimizationModel model = to_model(*m);
+  Constraint cut;
+  cut.name = "c";
+  cut.sense = sense == 'G' ? ConstraintSense::Ge : sense == 'E' ? ConstraintSense::Eq : ConstraintSense::Le;
+  cut.rhs = rhs;
+  for (int t = 0; t < nnz; ++t) cut.linear[var_name(col[t])] += val[t];
+  std::vector<std::unordered_map<std::string, double>> pts(static_cast<std::size_t>(n_points));
+  for (int k = 0; k < n_points; ++k)
+    for (int j = 0; j < m->n_vars; ++j) pts[k][var_name(j)] = points[static_cast<std::size_t>(k) * m->n_vars + j];
+  const std::string why = check_cut_validity(model, cut, pts, tol);
+  if (why.empty()) return 0;
+  return why.find("removes a known") != std::string::npos ? 1 : 2;
 }
 
 }  // extern "C"

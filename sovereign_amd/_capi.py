@@ -107,7 +107,8 @@ class BnbOptions(C.Structure):
         ("mip_gap", C.c_double), ("integer_tol", C.c_double), ("time_limit_seconds", C.c_double),
         ("candidate_order", CANDIDATE_ORDER_FN), ("candidate_order_user", C.c_void_p),
         ("branch_rule", C.c_int32), ("reliability_threshold", C.c_int32), ("max_strong_per_node", C.c_int32),
-        ("plunging", C.c_int32),
+        ("plunging", C.c_int32), ("max_cut_rounds", C.c_int32), ("max_cuts_per_node", C.c_int32),
+        ("cmir_at_root", C.c_int32), ("pad", C.c_int32),
     ]
 
 
@@ -117,11 +118,14 @@ class BnbResult(C.Structure):
         ("nodes", C.c_int64), ("lp_solves", C.c_int64), ("seconds", C.c_double),
         ("strong_branch_seconds", C.c_double), ("kernel_ms", C.c_double), ("node_lp_solves", C.c_int64),
         ("strong_lp_solves", C.c_int64), ("strong_lp_used", C.c_int64), ("node_lp_seconds", C.c_double),
-        ("max_depth", C.c_int32), ("node_lp_failures", C.c_int32),
+        ("max_depth", C.c_int32), ("node_lp_failures", C.c_int32), ("cuts_added", C.c_int32),
+        ("cuts_cover", C.c_int32), ("cuts_gomory", C.c_int32), ("cuts_cmir", C.c_int32), ("cut_rounds", C.c_int32),
+        ("cut_rejections", C.c_int32), ("root_bound_before_cuts", C.c_double), ("root_rows", C.c_int32),
+        ("pad2", C.c_int32),
     ]
 
     def as_dict(self):
-        return {name: getattr(self, name) for name, _ in self._fields_ if name != "pad"}
+        return {name: getattr(self, name) for name, _ in self._fields_ if not name.startswith("pad")}
 
 
 class LpModelC(C.Structure):
@@ -164,6 +168,14 @@ class QpResult(C.Structure):
         return {name: getattr(self, name) for name, _ in self._fields_ if name != "pad"}
 
 
+class CutList(C.Structure):
+    _fields_ = [
+        ("max_cuts", C.c_int32), ("n_cuts", C.c_int32), ("max_nnz", C.c_int64), ("nnz", C.c_int64),
+        ("row_ptr", _i32p), ("col", _i32p), ("val", _f64p), ("sense", C.c_char_p), ("rhs", _f64p),
+        ("family", _i32p), ("efficacy", _f64p),
+    ]
+
+
 class QpModelC(C.Structure):
     _fields_ = [
         ("n_vars", C.c_int32), ("n_rows", C.c_int32), ("col_ptr", _i32p), ("row_idx", _i32p), ("values", _f64p),
@@ -196,6 +208,10 @@ class QpIpmResult(C.Structure):
 _f32p = C.POINTER(C.c_float)
 
 SIGNATURES = {
+    "sov_separate_cuts": (C.c_int, [C.POINTER(LpModelC), C.POINTER(C.c_int8), _f64p, C.c_int32, C.c_double, C.c_int32,
+                                    C.c_int32, C.POINTER(CutList)]),
+    "sov_cut_validity": (C.c_int, [C.POINTER(LpModelC), C.c_int32, _i32p, _f64p, C.c_char, C.c_double, C.c_int32, _f64p,
+                                   C.c_double, C.POINTER(C.c_int32)]),
     "sov_dev_probe": (C.c_int, [C.POINTER(DevInfo)]),
     "sov_gpu_operations": (C.c_uint64, []),
     "sov_gpu_factorizations": (C.c_uint64, []),

@@ -74,12 +74,23 @@ def solve_lp(model: IO.Model) -> IO.Result:
 def bnb_options_from_env(model: IO.Model) -> dict:
     """MilpSolver::solve's options (milp_solver.cpp:31-98) for the parts built
     here: branch rule, strong-branching budget, reliability threshold, node
-    and time limits, plunging."""
+    and time limits, plunging, root cuts (SOVEREIGN_ENABLE_CUTS,
+    SOVEREIGN_MAX_CUT_ROUNDS with its size-based default of :46-60,
+    SOVEREIGN_MAX_CUTS_PER_NODE, SOVEREIGN_CMIR_MAX_DEPTH >= 0 turns cMIR on
+    at the root)."""
     n = model.n_vars
     rule = os.environ.get("SOVEREIGN_BRANCH_RULE", "strong")
     if rule not in milp.BRANCH_RULES:
         rule = "strong"
+    scale = n * max(1, len(model.con_names))
+    rounds_default = 1 if (scale > 500000 or n > 4000) else 2 if scale > 100000 else 5
+    rounds = max(0, _env_int("SOVEREIGN_MAX_CUT_ROUNDS", rounds_default))
+    if not _env_flag("SOVEREIGN_ENABLE_CUTS", True):
+        rounds = 0
     kw = {"branch_rule": rule,
+          "max_cut_rounds": rounds,
+          "max_cuts_per_node": max(0, _env_int("SOVEREIGN_MAX_CUTS_PER_NODE", 20)),
+          "cmir_at_root": _env_int("SOVEREIGN_CMIR_MAX_DEPTH", -1) >= 0,
           "max_strong_per_node": _env_int("SOVEREIGN_MAX_STRONG_PER_NODE", 2 if n > 1000 else 8),
           "reliability_threshold": _env_int("SOVEREIGN_RELIABILITY_THRESHOLD", 2 if n > 1000 else 4),
           "max_nodes": _env_int("SOVEREIGN_MAX_NODES", 100000),

@@ -15,6 +15,7 @@
 // taken from the strong-branching solves of the chosen variable.
 
 #include "internal.hpp"
+#include "cuts.hpp"
 #include "sovereign_bnb.h"
 
 #include <algorithm>
@@ -692,6 +693,10 @@ void sov_bnb_default_options(sov_bnb_options* o) {
   o->reliability_threshold = 4;
   o->max_strong_per_node = 8;
   o->plunging = 1;
+  o->max_cut_rounds = 5;
+  o->max_cuts_per_node = 20;
+  o->cmir_at_root = 0;
+  o->pad = 0;
 }
 
 int sov_strong_branch(const sov_lp_model* node, const double* node_lb, const double* node_ub, const int8_t* is_int,
@@ -766,9 +771,76 @@ int sov_bnb_solve(const sov_lp_model* milp, const int8_t* is_int, const sov_bnb_
   const auto t0 = Clock::now();
   const int nv = milp->n_vars;
   const std::vector<double> lb0(milp->lb, milp->lb + nv), ub0(milp->ub, milp->ub + nv);
-  const DenseForm F = dense_form(*milp, lb0.data(), ub0.data());
-  const int nslack = F.n - F.nv;
   const bool batched = bo->child_solver == SOV_CHILD_BATCHED && batched_form_ok(lb0.data(), nv);
+  int lp_solves = 0;
+
+  // Root cut rounds (apply_cuts at depth 0, branch_and_bound.cpp:1214-1273):
+  // separate at the root LP's point, gate, append, re-solve; a round whose
+  // LP does not solve is taken back and ends the loop (the reference keeps
+  // the pre-cut LP there). No incumbent exists yet, so the gate checks the
+  // bound box only. The tree then searches the model with the cuts.
+  res->root_bound_before_cuts = std::numeric_limits<double>::quiet_NaN();
+  sov::OwnedLpModel with_cuts;
+  const sov_lp_model* mdl = milp;
+  if (bo->max_cut_rounds > 0) {
+    std::vector<int8_t> kind(static_cast<std::size_t>(nv));
+    for (int j = 0; j < nv; ++j)
+      kind[j] = !is_int[j] ? sov::kContinuous : (lb0[j] >= 0.0 && ub0[j] <= 1.0 ? sov::kBinary : sov::kInteger);
+    sov::CutModel cm;
+    int rc = sov::cut_model_from(*milp, kind.data(), cm);
+    if (rc != SOV_OK) {
+      sov::set_error("sov_bnb_solve: malformed model");
+      return rc;
+    }
+    const std::size_t base_rows = cm.rows.size();
+    const std::size_t row_cap = batched ? 272 : std::numeric_limits<std::size_t>::max();
+    std::vector<double> x(static_cast<std::size_t>(nv));
+    sov_ipm_result r{};
+    if ((rc = sov_lp_solve(milp, opt, &r, x.data(), nullptr, nullptr)) < 0) return rc;
+    ++lp_solves;
+    if (r.status == 0) {
+      res->root_bound_before_cuts = r.objective;
+      std::vector<int> kept;  // family of each cut in the model
+      for (int round = 0; round < bo->max_cut_rounds && cm.rows.size() < row_cap; ++round) {
+        const std::size_t before = cm.rows.size();
+        std::vector<int> fam;
+        int rejected = 0;
+        sov::apply_cuts(cm, x.data(), bo->integer_tol, bo->max_cuts_per_node, {}, base_rows, bo->cmir_at_root != 0,
+                        &rejected, &fam);
+        res->cut_rejections += rejected;
+        if (cm.rows.size() > row_cap) {
+          cm.rows.resize(row_cap);
+          fam.resize(row_cap - before);
+        }
+        if (cm.rows.size() == before) break;
+        with_cuts.build(*milp, cm);
+        std::vector<double> xc(static_cast<std::size_t>(nv));
+        sov_ipm_result rc2{};
+        if ((rc = sov_lp_solve(&with_cuts.model, opt, &rc2, xc.data(), nullptr, nullptr)) < 0) return rc;
+        ++lp_solves;
+        if (rc2.status != 0) {
+          cm.rows.resize(before);
+          break;
+        }
+        kept.insert(kept.end(), fam.begin(), fam.end());
+        ++res->cut_rounds;
+        x = std::move(xc);
+      }
+      for (int f : kept) {
+        ++res->cuts_added;
+        if (f == sov::kCutCover) ++res->cuts_cover;
+        else if (f == sov::kCutGomory) ++res->cuts_gomory;
+        else ++res->cuts_cmir;
+      }
+      if (cm.rows.size() > base_rows) {
+        with_cuts.build(*milp, cm);
+        mdl = &with_cuts.model;
+      }
+    }
+  }
+  res->root_rows = mdl->n_rows;
+  const DenseForm F = dense_form(*mdl, lb0.data(), ub0.data());
+  const int nslack = F.n - F.nv;
   if (batched && F.m > 272) {
     sov::set_error("sov_bnb_solve: batched child LPs need at most 272 rows");
     return SOV_ERR_ARG;
@@ -784,16 +856,15 @@ int sov_bnb_solve(const sov_lp_model* milp, const int8_t* is_int, const sov_bnb_
     if (rc != SOV_OK) return rc;
   }
   double kernel_ms = 0.0;
-  int lp_solves = 0;
-  const NodePresolve presolve(*milp);
+  const NodePresolve presolve(*mdl);
   auto solve_set = [&](std::vector<ChildSpec> kids, std::vector<ChildOut>& outv) {
     presolve.mark(kids);
     if (batched) {
       int rc = solve_children(batch.get(), mat_id, F, kids, *opt, outv, &kernel_ms, &lp_solves);
       if (rc == SOV_OK) rc = prove_infeasible(presolve, kids, outv);
-      return rc != SOV_OK ? rc : finish_unconverged(*milp, kids, F, *opt, outv, &lp_solves);
+      return rc != SOV_OK ? rc : finish_unconverged(*mdl, kids, F, *opt, outv, &lp_solves);
     }
-    const int rc = solve_children_one_by_one(*milp, kids, F, *opt, outv, &lp_solves);
+    const int rc = solve_children_one_by_one(*mdl, kids, F, *opt, outv, &lp_solves);
     return rc != SOV_OK ? rc : prove_infeasible(presolve, kids, outv);
   };
   auto node_spec = [&](const std::vector<double>& lb, const std::vector<double>& ub) {

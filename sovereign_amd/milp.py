@@ -124,12 +124,15 @@ class BranchAndBoundSolver:
     "full_strong", "pseudocost" or "most_fractional".
     ranker (full_strong): optional callable(x, lb, ub) -> key per variable
     (higher first), e.g. the learned surrogate; None keeps the
-    closest-to-1/2 order."""
+    closest-to-1/2 order. Cuts: max_cut_rounds rounds at the root (0: none)
+    of at most max_cuts_per_node validated cover and Gomory rounding cuts
+    (and cMIR with cmir_at_root), inherited by the whole tree."""
 
     def __init__(self, strong_branch_candidates=64, child_solver=CHILD_BATCHED, max_nodes=100000,
                  mip_gap=1e-6, integer_tol=1e-6, time_limit_seconds=0.0, ranker=None,
                  options: InteriorPointOptions | None = None, branch_rule="strong", reliability_threshold=4,
-                 max_strong_per_node=8, plunging=True):
+                 max_strong_per_node=8, plunging=True, max_cut_rounds=5, max_cuts_per_node=20,
+                 cmir_at_root=False):
         self.lib = K.load()
         self.bo = K.BnbOptions()
         self.lib.sov_bnb_default_options(C.byref(self.bo))
@@ -139,6 +142,9 @@ class BranchAndBoundSolver:
         self.bo.reliability_threshold = reliability_threshold
         self.bo.max_strong_per_node = max_strong_per_node
         self.bo.plunging = int(bool(plunging))
+        self.bo.max_cut_rounds = max_cut_rounds
+        self.bo.max_cuts_per_node = max_cuts_per_node
+        self.bo.cmir_at_root = int(bool(cmir_at_root))
         self.bo.strong_branch_candidates = strong_branch_candidates
         self.bo.child_solver = child_solver
         self.bo.max_nodes = max_nodes

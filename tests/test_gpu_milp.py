@@ -300,3 +300,52 @@ def test_bnb_free_variable_takes_per_lp_path(gpu_api):
         out.append((res["status"], res["nodes"], res["lp_solves"]))
     assert out[0] == out[1]
     assert out[0][0] != 0
+
+
+def cover_prone_knapsack(n=30, seed=7):
+    """One knapsack row whose profit-to-weight ratio grows with the weight,
+    so that the LP vertex fills the heaviest items and violates the greedy
+    cover (a cover cut fires at the root)."""
+    rng = np.random.default_rng(seed)
+    w = rng.integers(5, 60, size=n).astype(float)
+    profit = w * (1.0 + 0.002 * np.argsort(np.argsort(w)))
+    return P.LpModel(n_vars=n, n_rows=1, col_ptr=np.arange(n + 1), row_idx=np.zeros(n), values=w, obj=profit,
+                     lb=np.zeros(n), ub=np.ones(n), row_sense=b"L", rhs=np.array([np.floor(0.35 * w.sum())]),
+                     maximize=True)
+
+
+@pytest.mark.parametrize("seed", [7, 8, 9, 11])
+def test_bnb_root_cover_cuts_keep_the_optimum(gpu_api, seed):
+    """Root cut rounds (apply_cuts, branch_and_bound.cpp:648-694 at depth 0)
+    add validated cover cuts that tighten the root bound; the optimum
+    (scipy's HiGHS, an independent solver) is the same with and without."""
+    model = cover_prone_knapsack(seed=seed)
+    is_int = np.ones(model.n_vars, dtype=np.int8)
+    opt, _ = scipy_optimum(model, is_int)
+    say(f"bnb cover-prone knapsack seed {seed}")
+    rc = MI.BranchAndBoundSolver(strong_branch_candidates=4).solve(model, is_int)
+    rn = MI.BranchAndBoundSolver(strong_branch_candidates=4, max_cut_rounds=0).solve(model, is_int)
+    say(f"  cuts {rc['cuts_added']} (cover {rc['cuts_cover']}) rounds {rc['cut_rounds']} rows {rc['root_rows']} "
+        f"root bound {rc['root_bound_before_cuts']:.6f} -> {rc['root_bound']:.6f}, nodes {rc['nodes']} vs {rn['nodes']}")
+    for r in (rc, rn):
+        assert r["status"] == 0 and abs(r["objective"] - opt) <= 1e-6 * max(1.0, abs(opt))
+    assert rc["cuts_cover"] >= 1 and rc["cuts_added"] == rc["cuts_cover"] + rc["cuts_gomory"] + rc["cuts_cmir"]
+    assert rc["root_rows"] == model.n_rows + rc["cuts_added"] and rn["cuts_added"] == 0
+    # maximisation: cuts lower the LP bound, never below the optimum
+    assert rc["root_bound"] <= rc["root_bound_before_cuts"] + 1e-7
+    assert rc["root_bound"] >= opt - 1e-6 * max(1.0, abs(opt))
+    assert np.isnan(rn["root_bound_before_cuts"])
+
+
+def test_bnb_cmir_root_cuts_on_reference_fixture(gpu_api):
+    """cMIR at the root (SOVEREIGN_CMIR_MAX_DEPTH >= 0 in the reference) on the
+    18 x 3 knapsack: cuts are added and the published optimum 102 stays."""
+    model = G.load_fixture("multi_knapsack_18x3")
+    exp = G.milp_expected()["multi_knapsack_18x3"]["objective"]
+    say("bnb multi_knapsack_18x3 with cMIR at the root")
+    r = MI.BranchAndBoundSolver(strong_branch_candidates=4, cmir_at_root=True).solve(model, model.is_int)
+    say(f"  cuts {r['cuts_added']} (cmir {r['cuts_cmir']}) rejections {r['cut_rejections']} rows {r['root_rows']} "
+        f"nodes {r['nodes']}")
+    assert r["status"] == 0 and abs(r["objective"] - exp) <= 1e-6 * max(1, abs(exp))
+    assert r["cuts_cmir"] >= 1 and r["root_bound"] <= r["root_bound_before_cuts"] + 1e-7
+    assert r["root_bound"] >= exp - 1e-6
