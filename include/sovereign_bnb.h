@@ -74,7 +74,10 @@ typedef struct {
   int32_t max_cut_rounds;        /* 5: cut rounds at the root (0: no cuts), as max_cut_rounds */
   int32_t max_cuts_per_node;     /* 20: cuts added per round */
   int32_t cmir_at_root;          /* 0: cMIR separation off, the reference's default (cmir_max_depth = -1) */
-  int32_t pad;
+  int32_t warm_start;            /* 1: batched child LPs start from their parent node LP's iterate */
+  double warm_mu;                /* 0.1: the parent's iterate is its first with mu below this (iterations
+                                  * per child LP on the knapsack trees: 0.3 -> 13-21% fewer than cold,
+                                  * 0.1 -> 13-28%, 1e-3 -> from 16% more to 28% fewer) */
 } sov_bnb_options;
 
 typedef struct {
@@ -104,6 +107,7 @@ typedef struct {
   double root_bound_before_cuts; /* root LP objective (model sense) before the first round; NaN without cuts */
   int32_t root_rows;           /* rows of the model the tree searches (base rows + cuts) */
   int32_t pad2;
+  int64_t lp_iterations;       /* interior-point iterations of every LP the tree solved */
 } sov_bnb_result;
 
 void sov_bnb_default_options(sov_bnb_options* opt);

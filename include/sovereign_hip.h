@@ -279,6 +279,23 @@ int sov_batch_add_matrix(sov_batch* h, int32_t m, int32_t n, const double* a_col
 int sov_batch_solve(sov_batch* h, int32_t count, const int32_t* matrix_id, const double* b_all,
                     const int64_t* b_off, const double* c_all, const double* u_all, const int64_t* x_off,
                     const sov_ipm_options* opt, sov_batch_result* res, double* x_all);
+/* sov_batch_solve with warm starts and final iterates, for child LPs that
+ * start from their parent's point (the interior-point counterpart of the
+ * dual simplex warm start from the parent's LpBasis, solver/lp/include/
+ * sovereign/dual_simplex.hpp:19-23, used by solve_node_lp, branch_and_bound.cpp:
+ * 437-516). An iterate is (x, w, z, v: n each, then y: m) in the batched
+ * form's units: x the shifted variables, w = u - x, z and v the multipliers
+ * of x >= 0 and x <= u, y the row multipliers. state_in (may be NULL): LP k
+ * starts from state_in + in_off[k] when in_off[k] >= 0 (components are
+ * moved at least 1e-10 u inside their bounds), else from the default point.
+ * state_out (may be NULL): when out_off[k] >= 0, LP k's first iterate whose
+ * complementarity mu = (x'z + w'v) / #bounds falls below save_mu (a point
+ * near the central path, what a warm start wants) is written to state_out +
+ * out_off[k]; its final iterate when none does or save_mu <= 0. */
+int sov_batch_solve_warm(sov_batch* h, int32_t count, const int32_t* matrix_id, const double* b_all,
+                         const int64_t* b_off, const double* c_all, const double* u_all, const int64_t* x_off,
+                         const sov_ipm_options* opt, sov_batch_result* res, double* x_all, const double* state_in,
+                         const int64_t* in_off, double* state_out, const int64_t* out_off, double save_mu);
 /* Duration of the last solve's kernel (HIP events on the library stream). */
 double sov_batch_last_kernel_ms(const sov_batch* h);
 int sov_batch_free(sov_batch* h);

@@ -108,7 +108,7 @@ class BnbOptions(C.Structure):
         ("candidate_order", CANDIDATE_ORDER_FN), ("candidate_order_user", C.c_void_p),
         ("branch_rule", C.c_int32), ("reliability_threshold", C.c_int32), ("max_strong_per_node", C.c_int32),
         ("plunging", C.c_int32), ("max_cut_rounds", C.c_int32), ("max_cuts_per_node", C.c_int32),
-        ("cmir_at_root", C.c_int32), ("pad", C.c_int32),
+        ("cmir_at_root", C.c_int32), ("warm_start", C.c_int32), ("warm_mu", C.c_double),
     ]
 
 
@@ -121,7 +121,7 @@ class BnbResult(C.Structure):
         ("max_depth", C.c_int32), ("node_lp_failures", C.c_int32), ("cuts_added", C.c_int32),
         ("cuts_cover", C.c_int32), ("cuts_gomory", C.c_int32), ("cuts_cmir", C.c_int32), ("cut_rounds", C.c_int32),
         ("cut_rejections", C.c_int32), ("root_bound_before_cuts", C.c_double), ("root_rows", C.c_int32),
-        ("pad2", C.c_int32),
+        ("pad2", C.c_int32), ("lp_iterations", C.c_int64),
     ]
 
     def as_dict(self):
@@ -246,6 +246,9 @@ SIGNATURES = {
                                _f64p, _f64p, _f64p]),
     "sov_batch_create": (C.c_int, [C.POINTER(_vp)]),
     "sov_batch_add_matrix": (C.c_int, [_vp, C.c_int32, C.c_int32, _f64p, _i32p]),
+    "sov_batch_solve_warm": (C.c_int, [_vp, C.c_int32, _i32p, _f64p, C.POINTER(C.c_int64), _f64p, _f64p,
+                                       C.POINTER(C.c_int64), C.POINTER(IpmOptions), C.POINTER(BatchResult), _f64p, _f64p,
+                                       C.POINTER(C.c_int64), _f64p, C.POINTER(C.c_int64), C.c_double]),
     "sov_batch_solve": (C.c_int, [_vp, C.c_int32, _i32p, _f64p, C.POINTER(C.c_int64), _f64p, _f64p,
                                   C.POINTER(C.c_int64), C.POINTER(IpmOptions), C.POINTER(BatchResult), _f64p]),
     "sov_batch_last_kernel_ms": (C.c_double, [_vp]),

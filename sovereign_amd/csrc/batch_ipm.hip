@@ -61,11 +61,13 @@ struct BatchDesc {
   int mat;
   int pad;
   long long b_off, x_off, ws_off;
+  long long in_off, out_off;  // warm start / final iterate (x, w, z, v, y) offsets, -1: none
 };
 
 struct BatchOpts {
   double tol, tau, pcg_tol;
   int max_iter, pcg_max;
+  double save_mu;  // the iterate written to state_out: the first with mu below this (<= 0: the final one)
 };
 
 }  // namespace
@@ -815,7 +817,8 @@ __global__ __launch_bounds__(kThreads) void batch_ipm_kernel(const BatchMatrix* 
                                                              const double* __restrict__ c_all,
                                                              const double* __restrict__ u_all, double* ws,
                                                              double* x_all, sov_batch_result_dev* out, BatchOpts o,
-                                                             double* trace, int trace_lp) {
+                                                             double* trace, int trace_lp,
+                                                             const double* __restrict__ state_in, double* state_out) {
   extern __shared__ __align__(16) unsigned char smem[];
   double* stage = reinterpret_cast<double*>(smem);           // kStage doubles
   double* red = stage + kStage;                              // 64 doubles
@@ -868,24 +871,36 @@ __global__ __launch_bounds__(kThreads) void batch_ipm_kernel(const BatchMatrix* 
     __syncthreads();
     const Lp& L = s_L;
     const int m = L.m, n = L.n;
-    // Starting point: x mid-box (or 1), w = u - x, z = v = 1, y = 0.
+    // Starting point: x mid-box (or 1), w = u - x, z = v = 1, y = 0; or the
+    // given warm start (x, w, z, v: n each, then y: m), kept strictly
+    // inside the bounds (every component of an active column at least
+    // 1e-10, w = u - x on bounded ones).
     double nb = 0.0;
+    const double* st = (state_in && dsc.in_off >= 0) ? state_in + dsc.in_off : nullptr;
     for (int j = tid; j < n; j += kThreads) {
       const double uj = L.u[j];
       double xj = 0.0, wj = 0.0, vj = 0.0, zj = 0.0;
       if (uj > 0.0) {
-        xj = uj < 1e29 ? 0.5 * uj : 1.0;
-        wj = uj < 1e29 ? uj - xj : 0.0;
-        zj = 1.0;
-        vj = uj < 1e29 ? 1.0 : 0.0;
-        nb += uj < 1e29 ? 1.0 : 0.0;
+        const bool bnd = uj < 1e29;
+        if (st) {
+          xj = bnd ? fmin(fmax(st[j], 1e-10 * uj), uj - 1e-10 * uj) : fmax(st[j], 1e-10);
+          wj = bnd ? uj - xj : 0.0;
+          zj = fmax(st[2 * n + j], 1e-10);
+          vj = bnd ? fmax(st[3 * n + j], 1e-10) : 0.0;
+        } else {
+          xj = bnd ? 0.5 * uj : 1.0;
+          wj = bnd ? uj - xj : 0.0;
+          zj = 1.0;
+          vj = bnd ? 1.0 : 0.0;
+        }
+        nb += bnd ? 1.0 : 0.0;
       }
       L.x[j] = xj;
       L.w[j] = wj;
       L.z[j] = zj;
       L.v[j] = vj;
     }
-    for (int i = tid; i < m; i += kThreads) L.y[i] = 0.0;
+    for (int i = tid; i < m; i += kThreads) L.y[i] = st ? st[4 * n + i] : 0.0;
     double nact = 0.0;
     for (int j = tid; j < n; j += kThreads) nact += active(L, j) ? 1.0 : 0.0;
     nact = bred<0>(nact, red);
@@ -908,6 +923,7 @@ __global__ __launch_bounds__(kThreads) void batch_ipm_kernel(const BatchMatrix* 
     int fp64_from = -1;
     double best = 1e300;
     int best_it = 0;
+    bool saved = false;  // the warm-start iterate has been written
     for (int it = 0; it < o.max_iter; ++it) {
       __syncthreads();
       long long t0 = clock64();
@@ -992,6 +1008,17 @@ __global__ __launch_bounds__(kThreads) void batch_ipm_kernel(const BatchMatrix* 
       }
       const double mu = xz / fmax(nact + nb, 1.0);
       note(it, 4, mu);
+      if (state_out && dsc.out_off >= 0 && !saved && mu < o.save_mu) {  // a centred point for warm starts
+        double* so = state_out + dsc.out_off;
+        for (int j = tid; j < n; j += kThreads) {
+          so[j] = L.x[j];
+          so[n + j] = L.w[j];
+          so[2 * n + j] = L.z[j];
+          so[3 * n + j] = L.v[j];
+        }
+        for (int i = tid; i < m; i += kThreads) so[4 * n + i] = L.y[i];
+        saved = true;
+      }
       // scaling D = (z/x + v/w)^-1
       for (int j = tid; j < n; j += kThreads) {
         double dj = 0.0;
@@ -1102,6 +1129,16 @@ __global__ __launch_bounds__(kThreads) void batch_ipm_kernel(const BatchMatrix* 
     __syncthreads();
     if (x_all)
       for (int j = tid; j < n; j += kThreads) x_all[dsc.x_off + j] = L.x[j];
+    if (state_out && dsc.out_off >= 0 && !saved) {
+      double* so = state_out + dsc.out_off;
+      for (int j = tid; j < n; j += kThreads) {
+        so[j] = L.x[j];
+        so[n + j] = L.w[j];
+        so[2 * n + j] = L.z[j];
+        so[3 * n + j] = L.v[j];
+      }
+      for (int i = tid; i < m; i += kThreads) so[4 * n + i] = L.y[i];
+    }
     if (tid == 0) {
       sov_batch_result_dev r;
       r.status = status;
@@ -1220,12 +1257,21 @@ int sov_batch_add_matrix(sov_batch* h, int32_t m, int32_t n, const double* a_col
 int sov_batch_solve(sov_batch* h, int32_t count, const int32_t* matrix_id, const double* b_all,
                     const int64_t* b_off, const double* c_all, const double* u_all, const int64_t* x_off,
                     const sov_ipm_options* opt, sov_batch_result* res, double* x_all) {
-  if (!h || count < 0 || !matrix_id || !b_all || !b_off || !c_all || !u_all || !x_off || !opt || !res)
+  return sov_batch_solve_warm(h, count, matrix_id, b_all, b_off, c_all, u_all, x_off, opt, res, x_all, nullptr,
+                              nullptr, nullptr, nullptr, 0.0);
+}
+
+int sov_batch_solve_warm(sov_batch* h, int32_t count, const int32_t* matrix_id, const double* b_all,
+                         const int64_t* b_off, const double* c_all, const double* u_all, const int64_t* x_off,
+                         const sov_ipm_options* opt, sov_batch_result* res, double* x_all, const double* state_in,
+                         const int64_t* in_off, double* state_out, const int64_t* out_off, double save_mu) {
+  if (!h || count < 0 || !matrix_id || !b_all || !b_off || !c_all || !u_all || !x_off || !opt || !res ||
+      (state_in && !in_off) || (state_out && !out_off))
     return SOV_ERR_ARG;
   if (count == 0) return SOV_OK;
   hipStream_t s = sov::stream();
   std::vector<BatchDesc> descs(static_cast<std::size_t>(count));
-  long long b_total = 0, x_total = 0, ws_total = 0;
+  long long b_total = 0, x_total = 0, ws_total = 0, in_total = 0, out_total = 0;
   int mmax = 0;
   for (int k = 0; k < count; ++k) {
     const int id = matrix_id[k];
@@ -1236,9 +1282,12 @@ int sov_batch_solve(sov_batch* h, int32_t count, const int32_t* matrix_id, const
     }
     const BatchMatrix& mt = h->mats[id];
     mmax = std::max(mmax, mt.m);
-    descs[k] = {id, 0, b_off[k], x_off[k], ws_total};
+    const long long io = state_in ? in_off[k] : -1, oo = state_out ? out_off[k] : -1;
+    descs[k] = {id, 0, b_off[k], x_off[k], ws_total, io < 0 ? -1 : io, oo < 0 ? -1 : oo};
     b_total = std::max<long long>(b_total, b_off[k] + mt.m);
     x_total = std::max<long long>(x_total, x_off[k] + mt.n);
+    if (io >= 0) in_total = std::max<long long>(in_total, io + 4LL * mt.n + mt.m);
+    if (oo >= 0) out_total = std::max<long long>(out_total, oo + 4LL * mt.n + mt.m);
     // 16 n-vectors, 9 m-vectors, the packed FP64 matrix, 128 doubles (256
     // floats) per 16-row block for the diagonal-block inverses, and ldr
     // floats for sqrt(D)
@@ -1255,8 +1304,12 @@ int sov_batch_solve(sov_batch* h, int32_t count, const int32_t* matrix_id, const
   auto* d_x = x_all ? tmp.get<double>(static_cast<std::size_t>(x_total)) : nullptr;
   auto* d_out = tmp.get<sov_batch_result_dev>(static_cast<std::size_t>(count));
   auto* d_queue = tmp.get<unsigned>(1);
-  if (!d_mats || !d_desc || !d_b || !d_c || !d_u || !d_ws || (x_all && !d_x) || !d_out || !d_queue)
+  double* d_sin = in_total > 0 ? tmp.get<double>(static_cast<std::size_t>(in_total)) : nullptr;
+  double* d_sout = out_total > 0 ? tmp.get<double>(static_cast<std::size_t>(out_total)) : nullptr;
+  if (!d_mats || !d_desc || !d_b || !d_c || !d_u || !d_ws || (x_all && !d_x) || !d_out || !d_queue ||
+      (in_total > 0 && !d_sin) || (out_total > 0 && !d_sout))
     return SOV_ERR_NOMEM;
+  if (d_sin) SOV_HIP_TRY(hipMemcpyAsync(d_sin, state_in, in_total * sizeof(double), hipMemcpyHostToDevice, s));
   SOV_HIP_TRY(hipMemcpyAsync(d_mats, h->mats.data(), h->mats.size() * sizeof(BatchMatrix), hipMemcpyHostToDevice, s));
   SOV_HIP_TRY(hipMemcpyAsync(d_desc, descs.data(), descs.size() * sizeof(BatchDesc), hipMemcpyHostToDevice, s));
   SOV_HIP_TRY(hipMemcpyAsync(d_b, b_all, b_total * sizeof(double), hipMemcpyHostToDevice, s));
@@ -1269,6 +1322,7 @@ int sov_batch_solve(sov_batch* h, int32_t count, const int32_t* matrix_id, const
   o.max_iter = opt->max_iterations > 0 ? std::min(opt->max_iterations, 200) : 100;
   o.pcg_tol = opt->refine_tol > 0 ? opt->refine_tol : 1e-12;
   o.pcg_max = opt->refine_max > 0 ? opt->refine_max : 30;
+  o.save_mu = save_mu;
   int dev = 0, cus = 0;
   SOV_HIP_TRY(hipGetDevice(&dev));
   SOV_HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
@@ -1288,7 +1342,7 @@ int sov_batch_solve(sov_batch* h, int32_t count, const int32_t* matrix_id, const
     SOV_HIP_TRY(hipMemsetAsync(d_trace, 0, sizeof(double) * kTraceRows * kTraceCols, s));
   }
   batch_ipm_kernel<<<grid, kThreads, lds_bytes(mmax), s>>>(d_mats, d_desc, count, d_queue, d_b, d_c, d_u, d_ws, d_x,
-                                                          d_out, o, d_trace, trace_lp);
+                                                          d_out, o, d_trace, trace_lp, d_sin, d_sout);
   const hipError_t le = hipGetLastError();
   (void)hipEventRecord(e1, s);
   if (le != hipSuccess) {
@@ -1298,6 +1352,7 @@ int sov_batch_solve(sov_batch* h, int32_t count, const int32_t* matrix_id, const
   std::vector<sov_batch_result_dev> host(static_cast<std::size_t>(count));
   SOV_HIP_TRY(hipMemcpyAsync(host.data(), d_out, count * sizeof(sov_batch_result_dev), hipMemcpyDeviceToHost, s));
   if (x_all) SOV_HIP_TRY(hipMemcpyAsync(x_all, d_x, x_total * sizeof(double), hipMemcpyDeviceToHost, s));
+  if (d_sout) SOV_HIP_TRY(hipMemcpyAsync(state_out, d_sout, out_total * sizeof(double), hipMemcpyDeviceToHost, s));
   SOV_HIP_TRY(hipStreamSynchronize(s));
   (void)hipEventElapsedTime(&h->last_ms, e0, e1);
   if (d_trace) {

@@ -135,9 +135,22 @@ double model_objective(const DenseForm& F, double scaled_min_obj) {
   return F.sense * (scaled_min_obj * F.cscale) + F.constant;
 }
 
+// A batched LP's iterate (x, w, z, v: n each, y: m, in the form's units)
+// and the lower bounds it was shifted by (model units, n): the warm start
+// of its children.
+struct WarmState {
+  std::vector<double> v, lb;
+};
+
 struct ChildSpec {
   std::vector<double> lb, ub;
   bool empty = false;
+  std::shared_ptr<const WarmState> warm;  // the parent LP's iterate, or none (default start)
+};
+
+struct WarmOpts {
+  bool on = false;
+  double save_mu = 0.1;
 };
 
 // Node presolve: proves a node LP infeasible from its rows' activity ranges
@@ -300,16 +313,52 @@ struct ChildOut {
   int iterations = 0;
   double objective = 0.0;  // model sense
   std::vector<double> x;   // structural values (unshifted)
+  std::shared_ptr<const WarmState> state;  // batched solves with warm starts on: the iterate its children start from
 };
+
+// The parent's iterate mapped into a child's shifted box: x moves by the
+// change of lower bound, components pushed at least `lift` (or half the
+// box) inside their bounds, their multipliers raised so that each
+// complementarity product is at least save_mu; w, z, v, y carried over.
+void warm_start_for(const DenseForm& F, const WarmState& W, const ChildSpec& kid, const std::vector<double>& u,
+                    double save_mu, double* out) {
+  const int n = F.n, m = F.m;
+  const double lift = std::sqrt(save_mu);
+  for (int j = 0; j < n; ++j) {
+    double x = W.v[j] + (W.lb[j] - kid.lb[j]) / F.colscale[j];
+    double w = 0.0, z = W.v[2 * n + j], v = W.v[3 * n + j];
+    const double uj = u[j];
+    if (uj <= 0.0) {
+      x = w = z = v = 0.0;
+    } else if (uj < 1e29) {
+      const double d = std::min(lift, 0.5 * uj);
+      x = std::min(std::max(x, d), uj - d);
+      w = uj - x;
+      z = std::max(z, save_mu / x);
+      v = std::max(v, save_mu / w);
+    } else {
+      x = std::max(x, lift);
+      z = std::max(z, save_mu / x);
+      v = 0.0;
+    }
+    out[j] = x;
+    out[n + j] = w;
+    out[2 * n + j] = z;
+    out[3 * n + j] = v;
+  }
+  for (int i = 0; i < m; ++i) out[4 * n + i] = W.v[4 * n + i];
+}
 
 // Solves a set of children of one node in one batched launch.
 int solve_children(sov_batch* batch, int mat_id, const DenseForm& F, const std::vector<ChildSpec>& kids,
-                   const sov_ipm_options& opt, std::vector<ChildOut>& out, double* kernel_ms, int* lp_solves) {
+                   const sov_ipm_options& opt, std::vector<ChildOut>& out, double* kernel_ms, int* lp_solves,
+                   const WarmOpts& warm = WarmOpts{}) {
   const int K = static_cast<int>(kids.size());
   out.assign(static_cast<std::size_t>(K), ChildOut{});
   std::vector<int> live;
-  std::vector<double> b_all, c_all, u_all, shift_obj;
-  std::vector<int64_t> b_off, x_off;
+  std::vector<double> b_all, c_all, u_all, shift_obj, s_in;
+  std::vector<int64_t> b_off, x_off, in_off, out_off;
+  const int state_len = 4 * F.n + F.m;
   for (int k = 0; k < K; ++k) {
     std::vector<double> b(static_cast<std::size_t>(F.m)), u(static_cast<std::size_t>(F.n));
     double so = 0.0;
@@ -324,14 +373,28 @@ int solve_children(sov_batch* batch, int mat_id, const DenseForm& F, const std::
     c_all.insert(c_all.end(), F.c.begin(), F.c.end());
     u_all.insert(u_all.end(), u.begin(), u.end());
     shift_obj.push_back(so);
+    if (warm.on) {
+      out_off.push_back(static_cast<int64_t>(out_off.size()) * state_len);
+      if (kids[k].warm) {
+        in_off.push_back(static_cast<int64_t>(s_in.size()));
+        s_in.resize(s_in.size() + static_cast<std::size_t>(state_len));
+        warm_start_for(F, *kids[k].warm, kids[k], u, warm.save_mu, s_in.data() + in_off.back());
+      } else {
+        in_off.push_back(-1);
+      }
+    }
   }
   if (live.empty()) return SOV_OK;
   const int L = static_cast<int>(live.size());
   std::vector<int32_t> mats(static_cast<std::size_t>(L), mat_id);
   std::vector<sov_batch_result> res(static_cast<std::size_t>(L));
   std::vector<double> x_all(c_all.size());
-  const int rc = sov_batch_solve(batch, L, mats.data(), b_all.data(), b_off.data(), c_all.data(), u_all.data(),
-                                 x_off.data(), &opt, res.data(), x_all.data());
+  std::vector<double> s_out(warm.on ? static_cast<std::size_t>(L) * state_len : 0);
+  const bool any_in = !s_in.empty();
+  const int rc = sov_batch_solve_warm(batch, L, mats.data(), b_all.data(), b_off.data(), c_all.data(), u_all.data(),
+                                      x_off.data(), &opt, res.data(), x_all.data(), any_in ? s_in.data() : nullptr,
+                                      any_in ? in_off.data() : nullptr, warm.on ? s_out.data() : nullptr,
+                                      warm.on ? out_off.data() : nullptr, warm.save_mu);
   if (rc != SOV_OK) return rc;
   if (kernel_ms) *kernel_ms += sov_batch_last_kernel_ms(batch);
   if (lp_solves) *lp_solves += L;
@@ -343,6 +406,12 @@ int solve_children(sov_batch* batch, int mat_id, const DenseForm& F, const std::
     o.x.resize(static_cast<std::size_t>(F.nv));
     const ChildSpec& kid = kids[live[q]];
     for (int j = 0; j < F.nv; ++j) o.x[j] = F.colscale[j] * x_all[x_off[q] + j] + kid.lb[j];
+    if (warm.on) {
+      auto st = std::make_shared<WarmState>();
+      st->v.assign(s_out.begin() + out_off[q], s_out.begin() + out_off[q] + state_len);
+      st->lb = kid.lb;
+      o.state = std::move(st);
+    }
   }
   return SOV_OK;
 }
@@ -696,7 +765,8 @@ void sov_bnb_default_options(sov_bnb_options* o) {
   o->max_cut_rounds = 5;
   o->max_cuts_per_node = 20;
   o->cmir_at_root = 0;
-  o->pad = 0;
+  o->warm_start = 1;
+  o->warm_mu = 0.1;
 }
 
 int sov_strong_branch(const sov_lp_model* node, const double* node_lb, const double* node_ub, const int8_t* is_int,
@@ -857,20 +927,30 @@ int sov_bnb_solve(const sov_lp_model* milp, const int8_t* is_int, const sov_bnb_
   }
   double kernel_ms = 0.0;
   const NodePresolve presolve(*mdl);
+  WarmOpts warm;
+  warm.on = bo->warm_start != 0;
+  warm.save_mu = bo->warm_mu > 0.0 ? bo->warm_mu : 0.1;
+  long long lp_iterations = 0;
   auto solve_set = [&](std::vector<ChildSpec> kids, std::vector<ChildOut>& outv) {
     presolve.mark(kids);
+    int rc;
     if (batched) {
-      int rc = solve_children(batch.get(), mat_id, F, kids, *opt, outv, &kernel_ms, &lp_solves);
+      rc = solve_children(batch.get(), mat_id, F, kids, *opt, outv, &kernel_ms, &lp_solves, warm);
       if (rc == SOV_OK) rc = prove_infeasible(presolve, kids, outv);
-      return rc != SOV_OK ? rc : finish_unconverged(*mdl, kids, F, *opt, outv, &lp_solves);
+      if (rc == SOV_OK) rc = finish_unconverged(*mdl, kids, F, *opt, outv, &lp_solves);
+    } else {
+      rc = solve_children_one_by_one(*mdl, kids, F, *opt, outv, &lp_solves);
+      if (rc == SOV_OK) rc = prove_infeasible(presolve, kids, outv);
     }
-    const int rc = solve_children_one_by_one(*mdl, kids, F, *opt, outv, &lp_solves);
-    return rc != SOV_OK ? rc : prove_infeasible(presolve, kids, outv);
+    for (const ChildOut& o : outv) lp_iterations += o.iterations;
+    return rc;
   };
-  auto node_spec = [&](const std::vector<double>& lb, const std::vector<double>& ub) {
+  auto node_spec = [&](const std::vector<double>& lb, const std::vector<double>& ub,
+                       std::shared_ptr<const WarmState> from = nullptr) {
     ChildSpec s;
     s.lb = lb;
     s.ub = ub;
+    s.warm = std::move(from);
     s.lb.resize(static_cast<std::size_t>(F.n), 0.0);
     s.ub.resize(static_cast<std::size_t>(F.n), 1e30);
     return s;
@@ -894,6 +974,7 @@ int sov_bnb_solve(const sov_lp_model* milp, const int8_t* is_int, const sov_bnb_
     bool branch_up = false;
     double branch_distance = 0.0, parent_obj = 0.0;
     std::shared_ptr<const ChildOut> solved;  // the LP, when strong branching solved it
+    std::shared_ptr<const WarmState> parent_state;  // the parent LP's iterate (warm start of this node's LP)
   };
   auto worse = [](const OpenNode& a, const OpenNode& b) { return a.bound != b.bound ? a.bound > b.bound : a.id > b.id; };
   std::priority_queue<OpenNode, std::vector<OpenNode>, decltype(worse)> open(worse);
@@ -963,7 +1044,7 @@ int sov_bnb_solve(const sov_lp_model* milp, const int8_t* is_int, const sov_bnb_
     } else {
       const auto tl = Clock::now();
       std::vector<ChildOut> r;
-      const int rc = solve_set({node_spec(lb, ub)}, r);
+      const int rc = solve_set({node_spec(lb, ub, node.parent_state)}, r);
       if (rc != SOV_OK) return rc;
       lp = std::move(r[0]);
       ++node_lps;
@@ -1011,6 +1092,7 @@ int sov_bnb_solve(const sov_lp_model* milp, const int8_t* is_int, const sov_bnb_
     PairSolver pairs = [&](const std::vector<int>& vars, std::vector<ChildOut>& outv) {
       std::vector<ChildSpec> kids;
       children_specs(lb, ub, vars, lp.x.data(), nslack, kids);
+      for (ChildSpec& k : kids) k.warm = lp.state;
       return solve_set(kids, outv);
     };
     Decision d;
@@ -1067,6 +1149,7 @@ int sov_bnb_solve(const sov_lp_model* milp, const int8_t* is_int, const sov_bnb_
       child.branch_distance = up ? ce - val : val - fl;
       child.parent_obj = lp_obj;
       if (d.have_children) child.solved = std::make_shared<const ChildOut>(up ? d.up : d.down);
+      child.parent_state = lp.state;
       if (bo->plunging && up == prefer_up) plunge.push_back(std::move(child));
       else open.push(std::move(child));
     }
@@ -1081,6 +1164,7 @@ int sov_bnb_solve(const sov_lp_model* milp, const int8_t* is_int, const sov_bnb_
   res->kernel_ms = kernel_ms;
   res->max_depth = max_depth;
   res->node_lp_failures = failures;
+  res->lp_iterations = lp_iterations;
   res->seconds = std::chrono::duration<double>(Clock::now() - t0).count();
   if (!has_inc) {
     res->status = any_error ? 6 : limit_hit ? 5 : 2;

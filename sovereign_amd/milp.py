@@ -126,13 +126,16 @@ class BranchAndBoundSolver:
     (higher first), e.g. the learned surrogate; None keeps the
     closest-to-1/2 order. Cuts: max_cut_rounds rounds at the root (0: none)
     of at most max_cuts_per_node validated cover and Gomory rounding cuts
-    (and cMIR with cmir_at_root), inherited by the whole tree."""
+    (and cMIR with cmir_at_root), inherited by the whole tree. warm_start:
+    batched child LPs start from their parent node LP's first iterate with
+    complementarity below warm_mu (the interior-point counterpart of the
+    reference's dual simplex warm start from the parent basis)."""
 
     def __init__(self, strong_branch_candidates=64, child_solver=CHILD_BATCHED, max_nodes=100000,
                  mip_gap=1e-6, integer_tol=1e-6, time_limit_seconds=0.0, ranker=None,
                  options: InteriorPointOptions | None = None, branch_rule="strong", reliability_threshold=4,
                  max_strong_per_node=8, plunging=True, max_cut_rounds=5, max_cuts_per_node=20,
-                 cmir_at_root=False):
+                 cmir_at_root=False, warm_start=True, warm_mu=0.1):
         self.lib = K.load()
         self.bo = K.BnbOptions()
         self.lib.sov_bnb_default_options(C.byref(self.bo))
@@ -145,6 +148,8 @@ class BranchAndBoundSolver:
         self.bo.max_cut_rounds = max_cut_rounds
         self.bo.max_cuts_per_node = max_cuts_per_node
         self.bo.cmir_at_root = int(bool(cmir_at_root))
+        self.bo.warm_start = int(bool(warm_start))
+        self.bo.warm_mu = warm_mu
         self.bo.strong_branch_candidates = strong_branch_candidates
         self.bo.child_solver = child_solver
         self.bo.max_nodes = max_nodes
