@@ -390,13 +390,17 @@ def tree_leg(model, is_int, ranker, nodes=40, per_lp_nodes=4):
     """Wall seconds per branch-and-bound node over a tree (not only the
     root): the knapsack searched for a fixed node budget with each
     strong-branching setup, the plain per-LP path on a smaller budget (each
-    of its nodes costs seconds). Then the same setups solve a small knapsack
+    of its nodes costs seconds); the batched full strong branching once with
+    children warm-started from their parent's iterate (the default) and once
+    cold. Then the same setups solve a small knapsack
     (20 x 4, seed 3) to optimality, where their incumbents must agree."""
     from sovereign_amd import milp
 
     setups = [("per_lp_full_64", per_lp_nodes, dict(branch_rule="full_strong", strong_branch_candidates=64,
                                                     child_solver=milp.CHILD_PER_LP)),
               ("batched_full_64", nodes, dict(branch_rule="full_strong", strong_branch_candidates=64)),
+              ("batched_full_64_cold", nodes, dict(branch_rule="full_strong", strong_branch_candidates=64,
+                                                   warm_start=False)),
               ("batched_reliability", nodes, dict(branch_rule="strong"))]
     if ranker is not None:
         setups.append(("batched_surrogate_top8", nodes, dict(branch_rule="full_strong", strong_branch_candidates=8,
@@ -405,11 +409,13 @@ def tree_leg(model, is_int, ranker, nodes=40, per_lp_nodes=4):
     for name, budget, kw in setups:
         r = milp.BranchAndBoundSolver(max_nodes=budget, **kw).solve(model, is_int)
         out[name] = {"nodes": r["nodes"], "wall_s": r["seconds"], "wall_s_per_node": r["seconds"] / max(1, r["nodes"]),
-                     "lp_solves": r["lp_solves"], "strong_lp_solves": r["strong_lp_solves"],
+                     "lp_solves": r["lp_solves"], "lp_iterations": r["lp_iterations"],
+                     "iterations_per_lp": r["lp_iterations"] / max(1, r["lp_solves"]),
+                     "root_cuts": r["cuts_added"], "strong_lp_solves": r["strong_lp_solves"],
                      "node_lp_solves": r["node_lp_solves"], "kernel_ms": r["kernel_ms"], "status": r["status"],
                      "incumbent": r["objective"] if r["status"] in (0, 1) else None}
         log(f"tree {name}: {r['nodes']} nodes {r['seconds']:.2f}s ({out[name]['wall_s_per_node'] * 1e3:.1f} ms/node), "
-            f"{r['lp_solves']} LP solves, status {r['status']}")
+            f"{r['lp_solves']} LP solves ({out[name]['iterations_per_lp']:.2f} iterations each), status {r['status']}")
     base = out["per_lp_full_64"]["wall_s_per_node"]
     for name, _, _ in setups[1:]:
         out[f"per_node_speedup_{name}_vs_per_lp_full"] = base / out[name]["wall_s_per_node"]
