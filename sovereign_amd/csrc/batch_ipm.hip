@@ -22,6 +22,7 @@
 //                        preconditioner and the FP64 product A (D (A^T p)).
 // Residuals, step lengths and the termination test are FP64.
 
+#include "batch_layout.hpp"
 #include "internal.hpp"
 
 #include <algorithm>
@@ -36,15 +37,20 @@ typedef double dbl4_t __attribute__((ext_vector_type(4)));
 
 namespace {
 
-// 8 wavefronts: 256 registers a lane, 224 B of scratch a lane. (At 1024
-// threads, 128 registers a lane, the kernel spilled over 1 KB a lane; that
-// size was preferred while the LDS factorization went a column at a time
-// and wanted every thread.)
+namespace lay = sov_batch_layout;
+using lay::kMaxRows;
+using lay::kStage;
+using lay::row_ld;
+
+// 8 wavefronts: 256 registers a lane, about 300 B of scratch a lane. (At
+// 1024 threads, 128 registers a lane, the kernel spilled over 1 KB a lane;
+// that size was preferred while the LDS factorization went a column at a
+// time and wanted every thread.)
 constexpr int kThreads = 512;
 static_assert(kThreads % 256 == 0 && kThreads <= 1024, "gemv_n slices the columns in 256-row groups");
 constexpr int kWaves = kThreads / 64;
-constexpr int kMaxRows = 272;
-constexpr int kStage = 1024;  // doubles of LDS staging (vectors, partial sums)
+static_assert(kWaves <= lay::kRed, "block reductions keep one partial per wavefront");
+static_assert(kThreads / 256 * 256 <= kStage && kMaxRows <= kStage, "staged vectors and partial sums fit kStage");
 constexpr int kTraceRows = 200, kTraceCols = 16;  // SOVEREIGN_BATCH_TRACE record
 
 struct BatchMatrix {
@@ -54,8 +60,6 @@ struct BatchMatrix {
   int ldr;             // n rounded up to a multiple of 32
   const double* a64r;  // row-major m x ldr (FP64, for A^T v)
 };
-
-__host__ __device__ constexpr int row_ld(int n) { return (n + 31) / 32 * 32; }
 
 struct BatchDesc {
   int mat;
@@ -224,16 +228,11 @@ __device__ void assemble32(const Lp& L, float* M, double reg) {
   // J0 + 3 <= I; the row block's operands are loaded and scaled once per
   // k-chunk for all of them.
   const int nt = (m + 15) / 16;
-  int nitems = 0;
-  for (int I = 0; I < nt; ++I) nitems += I / 4 + 1;
+  const int nitems = lay::assembly_items(nt);
   const flt4_t zero = {0.f, 0.f, 0.f, 0.f};
   for (int q = wv; q < nitems; q += kWaves) {
-    int I = 0, rem = q;
-    while (rem >= I / 4 + 1) {
-      rem -= I / 4 + 1;
-      ++I;
-    }
-    const int J0 = 4 * rem;
+    int I, J0;
+    lay::assembly_item(q, I, J0);
     const int nj = min(4, I + 1 - J0);
     const int ra = 16 * I + lrow;
     const bool va = ra < m;
@@ -463,7 +462,7 @@ __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
 }
 
-constexpr int kBlk = 16;  // block width of the blocked LDS factorization and inversion
+using lay::kBlk;  // block width of the blocked LDS factorization and inversion
 // the inversion gives each block row below a block column a wavefront slot
 constexpr int kRowsPerWave = ((kMaxRows + kBlk - 1) / kBlk - 1 + kWaves - 1) / kWaves;
 
@@ -692,8 +691,8 @@ __device__ void normal_matvec(const Lp& L, const double* p, double* q, double* s
 template <typename T>
 __device__ int solve_normal(const Lp& L, const T* M, double* stage, double* red, double tol, int max_it) {
   const int tid = threadIdx.x, m = L.m;
-  double* pv = stage;       // [0, m)  preconditioner vector (LDS)
-  double* tv = stage + 512;  // [512, 512+m) preconditioner temp (LDS)
+  double* pv = stage + lay::kStagePv;  // m doubles: preconditioner vector (LDS)
+  double* tv = stage + lay::kStageTv;  // m doubles: preconditioner temporary (LDS)
   // x0 = P rhs
   for (int i = tid; i < m; i += kThreads) pv[i] = L.rhs[i];
   precondition(L, M, pv, tv);
@@ -820,9 +819,9 @@ __global__ __launch_bounds__(kThreads) void batch_ipm_kernel(const BatchMatrix* 
                                                              double* trace, int trace_lp,
                                                              const double* __restrict__ state_in, double* state_out) {
   extern __shared__ __align__(16) unsigned char smem[];
-  double* stage = reinterpret_cast<double*>(smem);           // kStage doubles
-  double* red = stage + kStage;                              // 64 doubles
-  float* M = reinterpret_cast<float*>(red + 64);             // packed lower triangle
+  double* stage = reinterpret_cast<double*>(smem);                    // kStage doubles
+  double* red = reinterpret_cast<double*>(smem + lay::lds_off_red());  // kRed doubles
+  float* M = reinterpret_cast<float*>(smem + lay::lds_off_matrix());   // packed lower triangle
   __shared__ int s_lp;
   __shared__ Lp s_L;
   __shared__ float s_xkk[kBlk * (kBlk + 1)];  // inverse of the diagonal block being factored
@@ -850,23 +849,15 @@ __global__ __launch_bounds__(kThreads) void batch_ipm_kernel(const BatchMatrix* 
       W.c = c_all + dsc.x_off;
       W.u = u_all + dsc.x_off;
       double* wsp = ws + dsc.ws_off;
-      double** nv[] = {&W.x, &W.w, &W.z, &W.v, &W.rc, &W.ru, &W.D, &W.dx, &W.dw, &W.dz, &W.dv,
-                       &W.dxa, &W.dwa, &W.dza, &W.dva, &W.t};
-      for (double** pp : nv) {
-        *pp = wsp;
-        wsp += W.n;
-      }
-      double** mv[] = {&W.y, &W.rb, &W.dy, &W.r, &W.p, &W.q, &W.zz, &W.rhs, &W.sc};
-      for (double** pp : mv) {
-        *pp = wsp;
-        wsp += W.m;
-      }
-      W.M64 = wsp;  // m (m + 1) / 2
-      W.xs = reinterpret_cast<float*>(wsp + static_cast<long long>(W.m) * (W.m + 1) / 2);  // 256 per block
-      {  // ldr floats, 16-byte aligned for vector loads (the workspace holds 16 bytes of slack)
-        const std::uintptr_t e = reinterpret_cast<std::uintptr_t>(W.xs + kBlk * kBlk * ((W.m + kBlk - 1) / kBlk));
-        W.sd = reinterpret_cast<float*>((e + 15) & ~static_cast<std::uintptr_t>(15));
-      }
+      double** nv[lay::kNVectors] = {&W.x, &W.w, &W.z, &W.v, &W.rc, &W.ru, &W.D, &W.dx, &W.dw, &W.dz, &W.dv,
+                                     &W.dxa, &W.dwa, &W.dza, &W.dva, &W.t};
+      for (int k = 0; k < lay::kNVectors; ++k) *nv[k] = wsp + lay::n_vector(k, W.n);
+      double** mv[lay::kMVectors] = {&W.y, &W.rb, &W.dy, &W.r, &W.p, &W.q, &W.zz, &W.rhs, &W.sc};
+      for (int k = 0; k < lay::kMVectors; ++k) *mv[k] = wsp + lay::m_vector(k, W.m, W.n);
+      W.M64 = wsp + lay::off_m64(W.m, W.n);
+      W.xs = reinterpret_cast<float*>(wsp + lay::off_xs(W.m, W.n));
+      W.sd = reinterpret_cast<float*>(
+          lay::sd_address(reinterpret_cast<std::uintptr_t>(wsp + lay::off_xs(W.m, W.n) + lay::xs_doubles(W.m))));
     }
     __syncthreads();
     const Lp& L = s_L;
@@ -1171,9 +1162,7 @@ struct sov_batch {
 
 namespace {
 
-std::size_t lds_bytes(int mmax) {
-  return sizeof(double) * (kStage + 64) + sizeof(float) * static_cast<std::size_t>(mmax) * (mmax + 1) / 2;
-}
+using lay::lds_bytes;
 
 template <typename T>
 T* dev_alloc(sov_batch* h, std::size_t count) {
@@ -1288,11 +1277,7 @@ int sov_batch_solve_warm(sov_batch* h, int32_t count, const int32_t* matrix_id, 
     x_total = std::max<long long>(x_total, x_off[k] + mt.n);
     if (io >= 0) in_total = std::max<long long>(in_total, io + 4LL * mt.n + mt.m);
     if (oo >= 0) out_total = std::max<long long>(out_total, oo + 4LL * mt.n + mt.m);
-    // 16 n-vectors, 9 m-vectors, the packed FP64 matrix, 128 doubles (256
-    // floats) per 16-row block for the diagonal-block inverses, and ldr
-    // floats for sqrt(D)
-    ws_total += 16LL * mt.n + 9LL * mt.m + static_cast<long long>(mt.m) * (mt.m + 1) / 2 + 128LL * ((mt.m + 15) / 16) +
-                mt.ldr / 2 + 2;
+    ws_total += lay::ws_doubles(mt.m, mt.n);
   }
   TmpBuffers tmp;  // per-solve device buffers, freed at the end of the call
   auto* d_mats = tmp.get<BatchMatrix>(h->mats.size());
