@@ -155,6 +155,52 @@ typedef struct sov_ne sov_ne;
 #define SOV_PREC_FP32 1
 
 int sov_ne_create(sov_mat* a, sov_ne** out);
+
+/* The factorization the normal equations use. The reference's selector
+ * (interior_point.cpp:129-175) picks a dense LU or the AMD-ordered sparse
+ * LDL^T by size and fill; here:
+ *   SOV_NE_DENSE   the blocked dense Cholesky above;
+ *   SOV_NE_SPARSE  the supernodal multifrontal Cholesky: the reference's
+ *                  minimum-degree ordering (amd_ordering.cpp:12-130) and
+ *                  elimination-tree analysis (sparse_ldlt.cpp:19-103) once,
+ *                  then per factorization a level-scheduled multifrontal
+ *                  factorization on the GPU with the same pivot rule, FP64
+ *                  only (a FP32 request factors in FP64, solves are direct);
+ *   SOV_NE_AUTO    sparse when the pattern's mean degree is at most 64 and
+ *                  the ordering predicts a factor of at most a tenth of the
+ *                  dense triangle, or when the dense matrix would not fit in
+ *                  device memory; dense otherwise. The environment variable
+ *                  SOVEREIGN_NE_PATH=dense|sparse overrides AUTO. */
+#define SOV_NE_AUTO 0
+#define SOV_NE_DENSE 1
+#define SOV_NE_SPARSE 2
+int sov_ne_create_path(sov_mat* a, int32_t path, sov_ne** out);
+
+/* Symbolic figures of a sparse factorization (all zero for a dense one):
+ * nnz(L) with the diagonal, the reference's factor_flops (sum over columns
+ * of the squared count below the diagonal), supernodes, tree levels (one
+ * launch each per factorization and per solve sweep), largest front. */
+typedef struct {
+  int32_t sparse;
+  int32_t supernodes;
+  int32_t levels;
+  int32_t max_front;
+  int64_t nnz_l;
+  double factor_flops;
+  int64_t front_doubles;
+} sov_ne_info;
+int sov_ne_get_info(const sov_ne* h, sov_ne_info* info);
+
+/* Minimum-degree ordering alone (AMDOrdering::compute, amd_ordering.cpp:
+ * 12-130): a symmetric pattern (CSC of either triangle or both, n x n), the
+ * node of least (degree, index) eliminated first, nodes of degree above
+ * max(16, 10 sqrt(n)) last by (degree, index). perm (n): the node eliminated
+ * k-th; iperm (n) its inverse; predicted_nnz (may be NULL): the fill the
+ * elimination predicts. Host only (no device needed). Returns SOV_OK, or
+ * SOV_SINGULAR when fill_limit > 0 and the prediction passes it (the
+ * reference's fill_limit_exceeded). */
+int sov_min_degree_order(int32_t n, const int32_t* col_ptr, const int32_t* row_idx, int64_t fill_limit,
+                         int32_t* perm, int32_t* iperm, int64_t* predicted_nnz);
 /* factor(d): 0 ok, 1 breakdown. d_host has ncols entries. */
 int sov_ne_factor(sov_ne* h, const double* d_host, int32_t precision);
 /* solve(rhs) in place, nrows entries. */
@@ -184,6 +230,9 @@ typedef struct {
   int32_t refine_max;     /* 30: PCG iterations per solve at most */
   double refine_tol;      /* 1e-12: relative residual target of a solve (an FP32-preconditioned
                            * solve stops at max(refine_tol, min(1e-6, 1e-3 mu))) */
+  int32_t normal_path;    /* SOV_NE_AUTO (0): the normal equations' factorization for the one-shot
+                           * solves (sov_ipm_solve_form, sov_lp_solve); sov_ipm_create_path sets it
+                           * for a resident solver */
 } sov_ipm_options;
 
 typedef struct {
@@ -200,6 +249,7 @@ typedef struct {
   double seconds;            /* whole solve, device-synchronised */
   double factor_seconds;     /* assembly + Cholesky, from HIP events */
   double solve_seconds;      /* triangular solves + refinement, from HIP events */
+  int32_t sparse_factor;     /* 1: the normal equations were factored by the sparse path */
 } sov_ipm_result;
 
 void sov_ipm_default_options(sov_ipm_options* opt);
@@ -211,6 +261,8 @@ void sov_ipm_default_options(sov_ipm_options* opt);
  * must outlive the solver. */
 typedef struct sov_ipm sov_ipm;
 int sov_ipm_create(sov_mat* a, sov_ipm** out);
+/* Same with the normal equations' factorization chosen (SOV_NE_*). */
+int sov_ipm_create_path(sov_mat* a, int32_t path, sov_ipm** out);
 int sov_ipm_solve(sov_ipm* h, const double* b, const double* c, const sov_ipm_options* opt,
                   sov_ipm_result* res, double* x, double* y, double* s);
 int sov_ipm_free(sov_ipm* h);

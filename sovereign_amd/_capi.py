@@ -19,6 +19,9 @@ SOV_SINGULAR = 1
 SOV_ERR_ARG = -1
 SOV_PREC_FP64 = 0
 SOV_PREC_FP32 = 1
+SOV_NE_AUTO = 0
+SOV_NE_DENSE = 1
+SOV_NE_SPARSE = 2
 
 # Declared C entry points (name -> (restype, argtypes)); the CPU test suite
 # checks that the built library exports every one of them.
@@ -52,6 +55,7 @@ class IpmOptions(C.Structure):
         ("mixed_switch_mu", C.c_double),
         ("refine_max", C.c_int32),
         ("refine_tol", C.c_double),
+        ("normal_path", C.c_int32),
     ]
 
 
@@ -70,6 +74,22 @@ class IpmResult(C.Structure):
         ("seconds", C.c_double),
         ("factor_seconds", C.c_double),
         ("solve_seconds", C.c_double),
+        ("sparse_factor", C.c_int32),
+    ]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+class NeInfo(C.Structure):
+    _fields_ = [
+        ("sparse", C.c_int32),
+        ("supernodes", C.c_int32),
+        ("levels", C.c_int32),
+        ("max_front", C.c_int32),
+        ("nnz_l", C.c_int64),
+        ("factor_flops", C.c_double),
+        ("front_doubles", C.c_int64),
     ]
 
     def as_dict(self):
@@ -230,6 +250,10 @@ SIGNATURES = {
     "sov_spmv_dev": (C.c_int, [_vp, C.c_int32, _vp, _vp]),
     "sov_spmv_csc": (C.c_int, [C.c_int32, C.c_int32, _i32p, _i32p, _f64p, _f64p, _f64p, C.c_int32]),
     "sov_ne_create": (C.c_int, [_vp, C.POINTER(_vp)]),
+    "sov_ne_create_path": (C.c_int, [_vp, C.c_int32, C.POINTER(_vp)]),
+    "sov_ne_get_info": (C.c_int, [_vp, C.POINTER(NeInfo)]),
+    "sov_min_degree_order": (C.c_int, [C.c_int32, _i32p, _i32p, C.c_int64, _i32p, _i32p,
+                                       C.POINTER(C.c_int64)]),
     "sov_ne_factor": (C.c_int, [_vp, _f64p, C.c_int32]),
     "sov_ne_solve": (C.c_int, [_vp, _f64p]),
     "sov_ne_last_residual": (C.c_double, [_vp]),
@@ -237,6 +261,7 @@ SIGNATURES = {
     "sov_ne_free": (C.c_int, [_vp]),
     "sov_ipm_default_options": (None, [C.POINTER(IpmOptions)]),
     "sov_ipm_create": (C.c_int, [_vp, C.POINTER(_vp)]),
+    "sov_ipm_create_path": (C.c_int, [_vp, C.c_int32, C.POINTER(_vp)]),
     "sov_ipm_solve": (C.c_int, [_vp, _f64p, _f64p, C.POINTER(IpmOptions), C.POINTER(IpmResult),
                                 _f64p, _f64p, _f64p]),
     "sov_ipm_free": (C.c_int, [_vp]),

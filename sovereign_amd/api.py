@@ -129,16 +129,43 @@ class SparseMatrix:
             pass
 
 
+_NE_PATHS = {"auto": K.SOV_NE_AUTO, "dense": K.SOV_NE_DENSE, "sparse": K.SOV_NE_SPARSE}
+
+
+def min_degree_order(n, col_ptr, row_idx, fill_limit=0):
+    """AMDOrdering::compute (amd_ordering.cpp:12-130) on the host: returns
+    (perm, iperm, predicted_nnz), or None when fill_limit > 0 is passed."""
+    lib = K.load()
+    cp, ri = K.i32(col_ptr), K.i32(row_idx)
+    perm = np.zeros(n, dtype=np.int32)
+    iperm = np.zeros(n, dtype=np.int32)
+    pred = C.c_int64(0)
+    rc = K.check(lib.sov_min_degree_order(n, K.pi(cp), K.pi(ri), int(fill_limit), K.pi(perm), K.pi(iperm),
+                                          C.byref(pred)), "sov_min_degree_order")
+    if rc != K.SOV_OK:
+        return None
+    return perm, iperm, int(pred.value)
+
+
 class NormalEquations:
     """factor(d) / solve(rhs) of M = A diag(d) A^T + 1e-12 I on the GPU.
     precision: 'fp64' (direct FP64 Cholesky) or 'fp32' (FP32 Cholesky as the
-    preconditioner of FP64 conjugate gradients)."""
+    preconditioner of FP64 conjugate gradients). path: 'auto', 'dense' (blocked
+    dense Cholesky) or 'sparse' (minimum-degree ordered multifrontal Cholesky,
+    FP64 only), the reference's dense / sparse selector
+    (interior_point.cpp:129-175)."""
 
-    def __init__(self, A: SparseMatrix):
+    def __init__(self, A: SparseMatrix, path: str = "auto"):
         self._lib = K.load()
         self.A = A
         self.handle = C.c_void_p()
-        K.check(self._lib.sov_ne_create(A.handle, C.byref(self.handle)), "sov_ne_create")
+        K.check(self._lib.sov_ne_create_path(A.handle, _NE_PATHS[path], C.byref(self.handle)), "sov_ne_create_path")
+
+    def info(self) -> dict:
+        """Symbolic figures of the sparse factorization (sparse = 0 for the dense one)."""
+        inf = K.NeInfo()
+        K.check(self._lib.sov_ne_get_info(self.handle, C.byref(inf)), "sov_ne_get_info")
+        return inf.as_dict()
 
     def factor(self, d, precision: str = "fp64") -> bool:
         d = K.f64(d)
@@ -187,6 +214,7 @@ class InteriorPointOptions:
     mixed_switch_mu: float = 1e-6
     refine_max: int = 30
     refine_tol: float = 1e-12
+    normal_path: str = "auto"  # 'auto', 'dense' or 'sparse' (NormalEquations)
 
     def to_c(self) -> K.IpmOptions:
         o = K.IpmOptions()
@@ -199,6 +227,7 @@ class InteriorPointOptions:
         o.mixed_switch_mu = self.mixed_switch_mu
         o.refine_max = self.refine_max
         o.refine_tol = self.refine_tol
+        o.normal_path = _NE_PATHS[self.normal_path]
         return o
 
 
@@ -254,13 +283,14 @@ class ResidentIpm:
     runs the Mehrotra loop for (b, c). Used to time solves with the inputs
     already resident."""
 
-    def __init__(self, lp: FormLp):
+    def __init__(self, lp: FormLp, normal_path: str = "auto"):
         self._lib = K.load()
         _require_device()
         self.lp = lp
         self.A = SparseMatrix(lp.m, lp.n, lp.col_ptr, lp.row_idx, lp.values)
         self.handle = C.c_void_p()
-        K.check(self._lib.sov_ipm_create(self.A.handle, C.byref(self.handle)), "sov_ipm_create")
+        K.check(self._lib.sov_ipm_create_path(self.A.handle, _NE_PATHS[normal_path], C.byref(self.handle)),
+                "sov_ipm_create_path")
         self._b, self._c = K.f64(lp.b), K.f64(lp.c)
 
     def solve(self, options: InteriorPointOptions, want_solution: bool = False) -> dict:

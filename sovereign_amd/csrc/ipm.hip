@@ -371,7 +371,9 @@ int ipm_run(sov_ipm* h, const double* b_h, const double* c_h, const sov_ipm_opti
   for (int i = 0; i < m; ++i) bnorm = std::max(bnorm, std::abs(b_h[i]));
   for (int j = 0; j < n; ++j) cnorm = std::max(cnorm, std::abs(c_h[j]));
   const double tau = opt.fraction_to_boundary;
-  const bool mixed = opt.precision == SOV_PREC_FP32;
+  // The sparse factorization is FP64 with direct solves: no mixed scheme.
+  const bool mixed = opt.precision == SOV_PREC_FP32 && !ne.sparse_mode();
+  res->sparse_factor = ne.sparse_mode() ? 1 : 0;
   bool fp64_locked = !mixed;
   const double solve_tol = opt.refine_tol > 0 ? opt.refine_tol : 1e-12;
   const int solve_max = opt.refine_max > 0 ? opt.refine_max : 30;
@@ -629,10 +631,13 @@ void sov_ipm_default_options(sov_ipm_options* o) {
   o->mixed_switch_mu = 1e-6;
   o->refine_max = 30;
   o->refine_tol = 1e-12;
+  o->normal_path = SOV_NE_AUTO;
 }
 
-int sov_ipm_create(sov_mat* a, sov_ipm** out) {
-  if (!a || !out) return SOV_ERR_ARG;
+int sov_ipm_create(sov_mat* a, sov_ipm** out) { return sov_ipm_create_path(a, SOV_NE_AUTO, out); }
+
+int sov_ipm_create_path(sov_mat* a, int32_t path, sov_ipm** out) {
+  if (!a || !out || path < SOV_NE_AUTO || path > SOV_NE_SPARSE) return SOV_ERR_ARG;
   *out = nullptr;
   if (!sov::device_ready()) {
     sov::set_error("no usable GPU");
@@ -640,7 +645,7 @@ int sov_ipm_create(sov_mat* a, sov_ipm** out) {
   }
   auto* h = new sov_ipm;
   h->A = a;
-  const int rc = h->ne.init(a);
+  const int rc = h->ne.init(a, path);
   if (rc != SOV_OK) {
     delete h;
     return rc;
@@ -674,7 +679,7 @@ int sov_ipm_solve_form(int32_t m, int32_t n, const int32_t* col_ptr, const int32
   int rc = sov::mat_upload(m, n, col_ptr, row_idx, values, &A);
   if (rc != SOV_OK) return rc;
   sov_ipm* h = nullptr;
-  rc = sov_ipm_create(A, &h);
+  rc = sov_ipm_create_path(A, opt->normal_path, &h);
   if (rc == SOV_OK) rc = sov::ipm_run(h, b, c, *opt, res, x, y, s);
   sov_ipm_free(h);
   sov::mat_release(A);

@@ -158,7 +158,7 @@ extern "C" int sov_lp_solve(const sov_lp_model* model, const sov_ipm_options* op
   if (rc != SOV_OK) return rc;
   std::vector<double> x(static_cast<std::size_t>(F.n)), y(static_cast<std::size_t>(F.m)), s(static_cast<std::size_t>(F.n));
   sov_ipm* h = nullptr;
-  rc = sov_ipm_create(A, &h);
+  rc = sov_ipm_create_path(A, opt->normal_path, &h);
   if (rc == SOV_OK) rc = sov_ipm_solve(h, F.b.data(), F.c.data(), opt, res, x.data(), y.data(), s.data());
   sov_ipm_free(h);
   sov::mat_release(A);

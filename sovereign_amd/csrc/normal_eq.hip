@@ -17,6 +17,9 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <memory>
 #include <vector>
 
 namespace sov {
@@ -213,6 +216,8 @@ NormalSolver::~NormalSolver() {
   free_dev(tj_);
   free_dev(eoff_);
   free_dev(ediag_);
+  free_dev(slot_);
+  free_dev(ediag_perm_);
   free_dev(dense_a_);
   if (wbuf_) (void)hipFree(wbuf_);
   free_dev(M64_);
@@ -229,11 +234,16 @@ NormalSolver::~NormalSolver() {
   if (sc_host_) (void)hipHostFree(sc_host_);
 }
 
-int NormalSolver::init(sov_mat* a) {
+int NormalSolver::init(sov_mat* a, int path) {
   pcg_graphs_.clear();
   A_ = a;
   m_ = a->nrows;
   n_ = a->ncols;
+  if (path == SOV_NE_AUTO) {
+    const char* e = std::getenv("SOVEREIGN_NE_PATH");
+    if (e && std::strcmp(e, "dense") == 0) path = SOV_NE_DENSE;
+    if (e && std::strcmp(e, "sparse") == 0) path = SOV_NE_SPARSE;
+  }
   const std::vector<int32_t>& cp = a->h_cptr;
   const std::vector<int32_t>& ri = a->h_crow;
   // Plan size: pairs (p, q) of a column with row(p) >= row(q).
@@ -243,7 +253,7 @@ int NormalSolver::init(sov_mat* a) {
     triples += k * (k + 1) / 2;
   }
   const double dense_entries = static_cast<double>(m_) * m_;
-  if (static_cast<double>(triples) > 2.0 * dense_entries && m_ > 0) {
+  if (static_cast<double>(triples) > 2.0 * dense_entries && m_ > 0 && path != SOV_NE_SPARSE) {
     std::vector<double> Ad(static_cast<std::size_t>(m_) * n_, 0.0);
     for (int j = 0; j < n_; ++j)
       for (int p = cp[j]; p < cp[j + 1]; ++p) Ad[ri[p] + static_cast<std::size_t>(j) * m_] += a->h_cval[p];
@@ -302,6 +312,22 @@ int NormalSolver::init(sov_mat* a) {
       set_error("normal equations: plan upload failed");
       return SOV_ERR_NOMEM;
     }
+    if (path != SOV_NE_DENSE && m_ > 0) {
+      const int rc = init_sparse(eoff, path);
+      if (rc < 0) return rc;
+    }
+  }
+  if (!sp_ && path == SOV_NE_SPARSE && m_ > 0) {
+    set_error("normal equations: the sparse factorization was asked for and could not be set up");
+    return SOV_ERR_ARG;
+  }
+  if (sp_) {  // FP64 only: no dense matrix, no dense factor workspace
+    const std::size_t vec = static_cast<std::size_t>(std::max(std::max(m_, n_), 1));
+    for (double** v : {&r_, &z_, &p_, &q_, &t_, &b_}) SOV_HIP_TRY(hipMalloc(v, vec * sizeof(double)));
+    SOV_HIP_TRY(hipMalloc(&partial_, 2 * kRedBlocks * sizeof(double)));
+    SOV_HIP_TRY(hipMalloc(&sc_, sizeof(PcgScalars)));
+    SOV_HIP_TRY(hipHostMalloc(&sc_host_, sizeof(PcgScalars)));
+    return SOV_OK;
   }
   const std::size_t mm = static_cast<std::size_t>(m_) * m_;
   SOV_HIP_TRY(hipMalloc(&M64_, std::max<std::size_t>(mm, 1) * sizeof(double)));
@@ -313,6 +339,62 @@ int NormalSolver::init(sov_mat* a) {
   SOV_HIP_TRY(hipHostMalloc(&sc_host_, sizeof(PcgScalars)));
   int rc = ch64.reserve(std::max(m_, 1));
   if (rc != SOV_OK) return rc;
+  return SOV_OK;
+}
+
+// Sparse factorization set-up from the plan's entries (their dense offsets
+// row + col m, row >= col, give the lower pattern). AUTO keeps the dense
+// path when the pattern is dense-like (mean degree above 64), or when the
+// ordering predicts a factor above a tenth of the dense triangle and the
+// dense matrix fits the device; returns SOV_OK either way (sp_ set or not).
+int NormalSolver::init_sparse(const std::vector<long long>& eoff, int path) {
+  const long long entries = static_cast<long long>(eoff.size());
+  const double mean_degree = 2.0 * static_cast<double>(entries - m_) / m_;
+  std::size_t free_b = 0, total_b = 0;
+  if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) free_b = 0;
+  const double dense_bytes = 1.6 * 8.0 * static_cast<double>(m_) * m_;  // FP64 + FP32 matrices, factor workspace
+  const bool dense_fits = free_b == 0 || dense_bytes < 0.8 * static_cast<double>(free_b);
+  const double tri = 0.5 * static_cast<double>(m_) * (m_ + 1);
+  int64_t limit = 0;
+  if (path == SOV_NE_AUTO && dense_fits) {
+    if (mean_degree > 64.0 || m_ < 200) return SOV_OK;
+    limit = static_cast<int64_t>(0.1 * tri);
+  }
+  std::vector<int32_t> lcp(static_cast<std::size_t>(m_) + 1, 0), lri(static_cast<std::size_t>(entries));
+  for (long long e = 0; e < entries; ++e) ++lcp[static_cast<int>(eoff[e] / m_) + 1];  // column = offset / m
+  for (int j = 0; j < m_; ++j) lcp[j + 1] += lcp[j];
+  {
+    std::vector<int32_t> next(lcp.begin(), lcp.end() - 1);
+    for (long long e = 0; e < entries; ++e) lri[next[static_cast<int>(eoff[e] / m_)]++] = static_cast<int32_t>(eoff[e] % m_);
+  }
+  auto sp = std::make_unique<SparseCholesky>();
+  const int rc = sp->sym.analyse(m_, lcp, lri, limit);
+  if (rc == SOV_SINGULAR) return SOV_OK;  // the fill limit: dense
+  if (rc != SOV_OK) {
+    set_error("normal equations: sparse symbolic analysis failed");
+    return rc;
+  }
+  if (path == SOV_NE_AUTO && dense_fits && static_cast<double>(sp->sym.nnz_l) > 0.1 * tri) return SOV_OK;
+  std::vector<long long> slot(static_cast<std::size_t>(entries));
+  std::vector<int32_t> edp(static_cast<std::size_t>(entries));
+  for (long long e = 0; e < entries; ++e) {
+    const int col = static_cast<int>(eoff[e] / m_), row = static_cast<int>(eoff[e] % m_);
+    slot[e] = sp->sym.slot(row, col);
+    if (slot[e] < 0) {
+      set_error("normal equations: an entry outside the sparse factor's structure");
+      return SOV_ERR_ARG;
+    }
+    edp[e] = row == col ? sp->sym.iperm[row] : -1;
+  }
+  int r2 = sp->upload();
+  if (r2 != SOV_OK) return r2;
+  SOV_HIP_TRY(hipMalloc(&slot_, std::max<std::size_t>(slot.size(), 1) * sizeof(long long)));
+  SOV_HIP_TRY(hipMalloc(&ediag_perm_, std::max<std::size_t>(edp.size(), 1) * sizeof(int32_t)));
+  if (entries > 0) {
+    SOV_HIP_TRY(hipMemcpy(slot_, slot.data(), slot.size() * sizeof(long long), hipMemcpyHostToDevice));
+    SOV_HIP_TRY(hipMemcpy(ediag_perm_, edp.data(), edp.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+  }
+  sp_ = std::move(sp);
   return SOV_OK;
 }
 
@@ -343,6 +425,17 @@ int NormalSolver::factor(const double* d_dev, int precision, hipStream_t s) {
   d_ = d_dev;
   prec_ = precision;
   if (m_ == 0) return SOV_OK;
+  if (sp_) {
+    prec_ = SOV_PREC_FP64;
+    TimedPair* tp = timed_begin(s, kTimeAssemble);
+    int rc = sp_->zero_fronts(s);
+    if (rc != SOV_OK) return rc;
+    assemble_plan_kernel<double><<<blocks_for(entries_, 256), 256, 0, s>>>(
+        entries_, eptr_, ta_, tb_, tj_, slot_, ediag_perm_, A_->c_val, d_dev, sp_->fronts(), sp_->diag0(), kNormalReg);
+    SOV_LAUNCH_CHECK();
+    timed_end(tp, s, 2, 0.0);
+    return sp_->factor(s);
+  }
   if (precision == SOV_PREC_FP32) {
     if (!M32_) SOV_HIP_TRY(hipMalloc(&M32_, static_cast<std::size_t>(m_) * m_ * sizeof(float)));
     int rc = ch32.reserve(m_);
@@ -359,7 +452,7 @@ int NormalSolver::factor(const double* d_dev, int precision, hipStream_t s) {
 int NormalSolver::breakdown(hipStream_t s) {
   if (m_ == 0) return SOV_OK;
   FactorFlags f{};
-  FactorFlags* src = prec_ == SOV_PREC_FP32 ? ch32.flags() : ch64.flags();
+  FactorFlags* src = sp_ ? sp_->flags() : prec_ == SOV_PREC_FP32 ? ch32.flags() : ch64.flags();
   SOV_HIP_TRY(hipMemcpyAsync(&f, src, sizeof(f), hipMemcpyDeviceToHost, s));
   SOV_HIP_TRY(hipStreamSynchronize(s));
   replaced_ = f.replaced;
@@ -387,6 +480,7 @@ int NormalSolver::solve(double* x, hipStream_t s, double tol, int max_iter) {
   last_iters_ = 0;
   last_res_ = 0.0;
   if (m_ == 0) return SOV_OK;
+  if (sp_) return sp_->solve(x, s);
   if (prec_ != SOV_PREC_FP32) return ch64.solve(M64_, m_, x, nullptr, s);
   // Preconditioned CG on M x = b, preconditioner = the FP32 Cholesky factor.
   const unsigned rb = red_blocks(m_);
@@ -473,13 +567,15 @@ struct sov_ne {
 
 extern "C" {
 
-int sov_ne_create(sov_mat* a, sov_ne** out) {
-  if (!a || !out) return SOV_ERR_ARG;
+int sov_ne_create(sov_mat* a, sov_ne** out) { return sov_ne_create_path(a, SOV_NE_AUTO, out); }
+
+int sov_ne_create_path(sov_mat* a, int32_t path, sov_ne** out) {
+  if (!a || !out || path < SOV_NE_AUTO || path > SOV_NE_SPARSE) return SOV_ERR_ARG;
   *out = nullptr;
   if (!sov::device_ready()) return SOV_ERR_NODEV;
   auto* h = new sov_ne;
   h->A = a;
-  int rc = h->solver.init(a);
+  int rc = h->solver.init(a, path);
   if (rc == SOV_OK && hipMalloc(&h->d_dev, std::max(a->ncols, 1) * sizeof(double)) != hipSuccess) rc = SOV_ERR_NOMEM;
   if (rc == SOV_OK && hipMalloc(&h->x_dev, std::max(a->nrows, 1) * sizeof(double)) != hipSuccess) rc = SOV_ERR_NOMEM;
   if (rc != SOV_OK) {
@@ -522,6 +618,39 @@ int sov_ne_solve(sov_ne* h, double* rhs_host) {
 }
 
 double sov_ne_last_residual(const sov_ne* h) { return h ? h->last_residual : -1.0; }
+
+int sov_ne_get_info(const sov_ne* h, sov_ne_info* info) {
+  if (!h || !info) return SOV_ERR_ARG;
+  *info = sov_ne_info{};
+  const sov::SparseSymbolic* y = h->solver.sparse_symbolic();
+  if (!y) return SOV_OK;
+  info->sparse = 1;
+  info->supernodes = static_cast<int32_t>(y->first.size());
+  info->levels = static_cast<int32_t>(y->level_ptr.size()) - 1;
+  info->max_front = y->max_front;
+  info->nnz_l = y->nnz_l;
+  info->factor_flops = y->flops;
+  info->front_doubles = y->front_total;
+  return SOV_OK;
+}
+
+int sov_min_degree_order(int32_t n, const int32_t* col_ptr, const int32_t* row_idx, int64_t fill_limit,
+                         int32_t* perm, int32_t* iperm, int64_t* predicted_nnz) {
+  if (n < 0 || !col_ptr || (n > 0 && (!row_idx || !perm || !iperm))) return SOV_ERR_ARG;
+  for (int j = 0; j < n; ++j)
+    if (col_ptr[j + 1] < col_ptr[j]) return SOV_ERR_ARG;
+  for (int p = 0; p < col_ptr[n]; ++p)
+    if (row_idx[p] < 0 || row_idx[p] >= n) return SOV_ERR_ARG;
+  std::vector<int> pv, ipv;
+  int64_t pred = 0;
+  const bool ok = sov::min_degree_order(n, col_ptr, row_idx, fill_limit, pv, ipv, &pred);
+  if (predicted_nnz) *predicted_nnz = pred;
+  for (int k = 0; k < n; ++k) {
+    perm[k] = pv[k];
+    iperm[k] = ipv[k];
+  }
+  return ok ? SOV_OK : SOV_SINGULAR;
+}
 int32_t sov_ne_replaced_pivots(const sov_ne* h) { return h ? h->solver.replaced_pivots() : 0; }
 
 int sov_ne_free(sov_ne* h) {

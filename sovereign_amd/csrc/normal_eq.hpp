@@ -4,6 +4,9 @@
 
 #include "chol.hpp"
 #include "internal.hpp"
+#include "sparse_chol.hpp"
+
+#include <memory>
 
 namespace sov {
 
@@ -17,7 +20,10 @@ struct PcgScalars {
 class NormalSolver {
  public:
   ~NormalSolver();
-  int init(sov_mat* a);
+  // path: SOV_NE_AUTO (sparse when the ordered factor stays sparse; the
+  // SOVEREIGN_NE_PATH environment variable, dense / sparse, overrides it),
+  // SOV_NE_DENSE or SOV_NE_SPARSE.
+  int init(sov_mat* a, int path = SOV_NE_AUTO);
   // factor(d) with d on the device. Asynchronous; check breakdown().
   int factor(const double* d_dev, int precision, hipStream_t s);
   // Waits for the last factorization and reports 0 ok, 1 breakdown.
@@ -33,6 +39,9 @@ class NormalSolver {
   double last_residual() const { return last_res_; }
   int replaced_pivots() const { return replaced_; }
   int dense_mode() const { return dense_a_ != nullptr; }
+  // The sparse factorization is in use (every factorization FP64, direct solves).
+  bool sparse_mode() const { return sp_ != nullptr; }
+  const SparseSymbolic* sparse_symbolic() const { return sp_ ? &sp_->sym : nullptr; }
   DenseCholesky<double> ch64;
   DenseCholesky<float> ch32;
   double assemble_ms = 0.0;
@@ -41,6 +50,7 @@ class NormalSolver {
   template <typename T>
   int assemble(const double* d_dev, T* M, hipStream_t s);
   int matvec(const double* p, double* q, hipStream_t s);  // q = M p in FP64, matrix-free
+  int init_sparse(const std::vector<long long>& eoff, int path);
 
   sov_mat* A_ = nullptr;
   int m_ = 0, n_ = 0;
@@ -54,6 +64,11 @@ class NormalSolver {
   int32_t* tj_ = nullptr;
   long long* eoff_ = nullptr;
   int32_t* ediag_ = nullptr;  // row index for diagonal entries, -1 otherwise
+  // Sparse factorization: each plan entry's slot in the fronts, and the
+  // permuted row of diagonal entries (-1 otherwise).
+  std::unique_ptr<SparseCholesky> sp_;
+  long long* slot_ = nullptr;
+  int32_t* ediag_perm_ = nullptr;
   // Dense-A plan (when the sparse plan would exceed the dense matrix):
   double* dense_a_ = nullptr;  // m x n column-major
   void* wbuf_ = nullptr;       // A diag(sqrt d), T = factor precision
