@@ -30,7 +30,7 @@ namespace {
 std::atomic<int> g_timing{0};  // 0 off, 1 every kernel class, 2 the bulk SYRK only
 
 constexpr int kPanelLd = kPanel + 1;  // LDS leading dimension of a panel block
-constexpr int kPanelBuffers = 3;      // L21 buffers in rotation (see factor)
+constexpr int kPanelBuffers = 3;      // L21 buffers of panel pairs in rotation (see factor)
 
 // ---------------------------------------------------------------------------
 // Diagonal block: Cholesky of the nb x nb block (nb <= kPanel = 128) and the
@@ -794,7 +794,7 @@ int DenseCholesky<T>::reserve(int m) {
   work_ = nullptr;
   flags_ = nullptr;
   solve_graphs_.clear();  // they hold the old buffers' addresses
-  SOV_HIP_TRY(hipMalloc(&panel_, sizeof(T) * static_cast<std::size_t>(m) * kPanel * kPanelBuffers));
+  SOV_HIP_TRY(hipMalloc(&panel_, sizeof(T) * static_cast<std::size_t>(m) * 2 * kPanel * kPanelBuffers));
   const std::size_t nsb = static_cast<std::size_t>((m + kSuper - 1) / kSuper);
   const std::size_t sq = static_cast<std::size_t>(kSuper) * kSuper;
   // Blocks above the diagonal of each super-block stay zero: nothing writes them.
@@ -896,6 +896,43 @@ int autotune_shapes(hipStream_t s) {
   return SOV_OK;
 }
 
+// Launches the diagonal-block kernel of the block at k0 (width nb); its
+// inverse goes to diagonal block (k0 % kSuper) / kPanel of super-block
+// k0 / kSuper. Returns that inverse.
+template <typename T>
+T* launch_panel(T* a, long long lda, int k0, int nb, const double* diag0, FactorFlags* flags, T* sup_inv,
+                hipStream_t s) {
+  T* pinv = sup_inv + static_cast<long long>(k0 / kSuper) * kSuper * kSuper +
+            static_cast<long long>(k0 % kSuper) * (kSuper + 1);
+  TimedPair* tk = timed_kernel(sizeof(T) == 8 ? kTimePotrf64 : kTimePotrf32);
+  if (tk) {  // the launch records its own start and stop events
+    tk->launches = 1;
+    tk->flops = 2.0 * nb * static_cast<double>(nb) * nb / 3.0;  // factor nb^3/3 + inverse nb^3/3
+    hipExtLaunchKernelGGL(potrf_panel_kernel<T>, dim3(1), dim3(kPanelThreads), panel_lds_bytes<T>(), s, tk->a, tk->b, 0,
+                          a, lda, k0, nb, diag0, flags, pinv, kSuper);
+  } else {
+    potrf_panel_kernel<T><<<1, kPanelThreads, panel_lds_bytes<T>(), s>>>(a, lda, k0, nb, diag0, flags, pinv, kSuper);
+  }
+  return pinv;
+}
+
+// Right-looking factorization by panel pairs. Pair q holds the column
+// blocks A = [k0, k0 + 128) and B = [k0 + 128, k0 + 256), k0 = 256 q:
+//   on s (the critical path): the diagonal block of A, the TRSM of A
+//     (L21 of A into the pair's buffer), strip 1 (block B updated by A),
+//     the diagonal block of B, the TRSM of B, then strip 2: the next pair's
+//     two column blocks [k0 + 256, k0 + 512) updated by both panels at once
+//     (K = 256);
+//   on side_: L21 of both panels copied back into the factor, then the bulk
+//     update of the columns from k0 + 512 by both panels (K = 256), which
+//     overlaps the next pair's chain.
+// Applying two panels per pass halves the reads and writes of the trailing
+// matrix (at K = 128 they cost about as much as the MFMA work). Each column
+// block is updated in panel order: strip 2 of pair q waits for the bulk of
+// pair q - 1 (the columns from k0 + 256); strip 1 follows strip 2 of the
+// previous pair on s. The buffer of pair q is written again by the TRSM of
+// pair q + 3, after strip 2 of pair q + 2, which waited for the bulk of
+// pair q + 1, queued on side_ after everything of pair q.
 template <typename T>
 int DenseCholesky<T>::factor(T* a, int m, const double* diag0, hipStream_t s) {
   int rc = reserve(m);
@@ -916,77 +953,92 @@ int DenseCholesky<T>::factor(T* a, int m, const double* diag0, hipStream_t s) {
   SOV_HIP_TRY(hipStreamWaitEvent(side_, ev_bulk_, 0));
   hipEvent_t bulk_done = ev_bulk_;  // completion of the last bulk update queued on the side stream
   const long long lda = m;
-  const std::size_t pstride = static_cast<std::size_t>(m) * kPanel;
-  for (int k0 = 0, p = 0; k0 < m; k0 += kPanel, ++p) {
-    const int nb = std::min(kPanel, m - k0);
-    const int below = m - k0 - nb;
-    // the panel inverse is the diagonal block p % 8 of super-block p / 8
-    T* pinv = sup_inv_ + static_cast<long long>(k0 / kSuper) * kSuper * kSuper +
-              static_cast<long long>(k0 % kSuper) * (kSuper + 1);
-    T* L21 = panel_ + (p % kPanelBuffers) * pstride;
+  const long long bstride = 2LL * kPanel * m;  // one pair buffer: m rows x 256 columns, leading dimension m
+  for (int k0 = 0, q = 0; k0 < m; k0 += 2 * kPanel, ++q) {
+    // buffer row r holds global row k0 + 128 + r; columns 0..127 L21 of A,
+    // 128..255 L21 of B (from global row k0 + 256, buffer row 128)
+    T* buf = panel_ + (q % kPanelBuffers) * bstride;
+    const int nbA = std::min(kPanel, m - k0);
+    const int belowA = m - k0 - nbA;
     TimedPair* tp = timed_begin(s, kTimePanel);
-    TimedPair* tk = timed_kernel(sizeof(T) == 8 ? kTimePotrf64 : kTimePotrf32);
-    if (tk) {  // the launch records its own start and stop events
-      tk->launches = 1;
-      tk->flops = 2.0 * nb * static_cast<double>(nb) * nb / 3.0;  // factor nb^3/3 + inverse nb^3/3
-      hipExtLaunchKernelGGL(potrf_panel_kernel<T>, dim3(1), dim3(kPanelThreads), panel_lds_bytes<T>(), s, tk->a, tk->b,
-                            0, a, lda, k0, nb, diag0, flags_, pinv, kSuper);
-    } else {
-      potrf_panel_kernel<T><<<1, kPanelThreads, panel_lds_bytes<T>(), s>>>(a, lda, k0, nb, diag0, flags_, pinv, kSuper);
-    }
+    T* pinvA = launch_panel<T>(a, lda, k0, nbA, diag0, flags_, sup_inv_, s);
     SOV_LAUNCH_CHECK();
-    if (below > 0) {
-      // L21 = A21 inv(L11)^T on MFMA, out of place (the updates read it
-      // there; the copy back into the factor runs on the side stream).
-      const T* a21 = a + (k0 + nb) + static_cast<long long>(k0) * lda;
-      SOV_HIP_TRY((gemm_update_shape<T, false, false, true>(panel_shape, below, nb, nb, a21, lda, pinv, kSuper, L21,
-                                                            lda, T(1), s)));
+    if (belowA > 0) {
+      // L21 = A21 inv(L11)^T on MFMA, out of place (the updates read it in
+      // the buffer; the copy back into the factor runs on the side stream)
+      const T* a21 = a + (k0 + nbA) + static_cast<long long>(k0) * lda;
+      SOV_HIP_TRY((gemm_update_shape<T, false, false, true>(panel_shape, belowA, nbA, nbA, a21, lda, pinvA, kSuper,
+                                                            buf, lda, T(1), s)));
     }
-    timed_end(tp, s, below > 0 ? 2 : 1, 0.0);
-    if (below <= 0) continue;
-    // Look-ahead: the next panel's column strip first (on s, the critical
-    // path), the rest of the trailing matrix on the side stream, where it
-    // overlaps the next panel's factorization. The strip update also writes
-    // the strictly upper part of its diagonal block, which nothing reads.
-    const int strip = std::min(kPanel, below);
-    const int rest = below - strip;
-    SOV_HIP_TRY(hipStreamWaitEvent(s, bulk_done, 0));  // the strip was part of the previous bulk update
-    TimedPair* ts = timed_begin(s, kTimeStrip);
-    T* a_strip = a + (k0 + nb) + static_cast<long long>(k0 + nb) * lda;
-    // ev_panel_ (the side stream's start, below) is recorded by the strip
-    // launch itself: a separate marker here would sit between the strip and
-    // the next panel kernel on the critical path.
-    SOV_HIP_TRY((gemm_update_shape<T, false, false>(panel_shape, below, strip, nb, L21, lda, L21, lda, a_strip, lda,
-                                                    T(-1), s, ts ? nullptr : ev_panel_)));
-    timed_end(ts, s, 1, 0.0);
-    if (ts) SOV_HIP_TRY(hipEventRecord(ev_panel_, s));
-    // Side stream: the bulk update of the rest, then L21 back into the
-    // factor (only the triangular solves read it there). bulk_done marks the
-    // bulk update: the next strip waits for it. The L21 buffer of this panel
-    // is written again kPanelBuffers = 3 panels later, by which time the main
-    // stream has waited for the next panel's bulk update, queued after this
-    // copy on the side stream.
+    timed_end(tp, s, belowA > 0 ? 2 : 1, 0.0);
+    if (belowA <= 0) break;
+    // strip 1: block B by panel A (it also writes the strictly upper part of
+    // B's diagonal block, which nothing reads)
+    const int nbB = std::min(kPanel, belowA);
+    const int belowB = belowA - nbB;
+    {
+      TimedPair* ts = timed_begin(s, kTimeStrip);
+      T* a_b = a + (k0 + nbA) + static_cast<long long>(k0 + nbA) * lda;
+      SOV_HIP_TRY((gemm_update_shape<T, false, false>(panel_shape, belowA, nbB, nbA, buf, lda, buf, lda, a_b, lda,
+                                                      T(-1), s)));
+      timed_end(ts, s, 1, 0.0);
+    }
+    const int kB = k0 + nbA;
+    tp = timed_begin(s, kTimePanel);
+    T* pinvB = launch_panel<T>(a, lda, kB, nbB, diag0, flags_, sup_inv_, s);
+    SOV_LAUNCH_CHECK();
+    T* bufB = buf + kPanel + kPanel * lda;  // column 128, row 128 (global row kB + nbB = k0 + 256)
+    if (belowB > 0) {
+      const T* a21 = a + (kB + nbB) + static_cast<long long>(kB) * lda;
+      SOV_HIP_TRY((gemm_update_shape<T, false, false, true>(panel_shape, belowB, nbB, nbB, a21, lda, pinvB, kSuper,
+                                                            bufB, lda, T(1), s)));
+    }
+    timed_end(tp, s, belowB > 0 ? 2 : 1, 0.0);
+    // strip 2 on s: the next pair's columns by both panels (K = 256), after
+    // the previous pair's bulk update; its launch records ev_panel_, which
+    // starts the side stream's work of this pair (released earlier, the bulk
+    // update's workgroups take every CU and the chain waits behind them)
+    const int w2 = std::min(2 * kPanel, belowB);
+    const int rest = belowB - w2;
+    const T* lp = buf + kPanel;  // rows from global k0 + 256, K = nbA + nbB = 256 columns
+    if (belowB > 0) {
+      SOV_HIP_TRY(hipStreamWaitEvent(s, bulk_done, 0));
+      TimedPair* ts = timed_begin(s, kTimeStrip);
+      T* a_c = a + (kB + nbB) + static_cast<long long>(kB + nbB) * lda;
+      SOV_HIP_TRY((gemm_update_shape<T, false, false>(panel_shape, belowB, w2, nbA + nbB, lp, lda, lp, lda, a_c, lda,
+                                                      T(-1), s, ts ? nullptr : ev_panel_)));
+      timed_end(ts, s, 1, 0.0);
+      if (ts) SOV_HIP_TRY(hipEventRecord(ev_panel_, s));
+    } else {
+      SOV_HIP_TRY(hipEventRecord(ev_panel_, s));
+    }
+    // side stream: both L21 back into the factor (only the triangular solves
+    // read them there), then the bulk update, which records its completion
+    // itself (bulk_done: ev_bulk_, or the stop event of its timed pair)
     SOV_HIP_TRY(hipStreamWaitEvent(side_, ev_panel_, 0));
-    // The bulk launch records its completion itself (bulk_done: ev_bulk_, or
-    // the stop event of its timed pair), so no marker follows it.
+    copy_panel_kernel<T><<<blocks_for(static_cast<long long>(belowA) * nbA, 256), 256, 0, side_>>>(buf, a, lda, k0 + nbA,
+                                                                                                    k0, belowA, nbA);
+    SOV_LAUNCH_CHECK();
+    if (belowB > 0) {
+      copy_panel_kernel<T><<<blocks_for(static_cast<long long>(belowB) * nbB, 256), 256, 0, side_>>>(
+          bufB, a, lda, kB + nbB, kB, belowB, nbB);
+      SOV_LAUNCH_CHECK();
+    }
     if (rest > 0) {
-      T* a22 = a + (k0 + nb + strip) + static_cast<long long>(k0 + nb + strip) * lda;
-      const T* l21r = L21 + strip;
+      T* a22 = a + (kB + nbB + w2) + static_cast<long long>(kB + nbB + w2) * lda;
+      const T* l3 = lp + w2;
       TimedPair* tb = timed_kernel(sizeof(T) == 8 ? kTimeSyrk : kTimeSyrk32);
       bulk_done = tb ? tb->b : ev_bulk_;
-      SOV_HIP_TRY((gemm_update_shape<T, false, true>(syrk_shape, rest, rest, nb, l21r, lda, l21r, lda, a22, lda, T(-1),
-                                                     side_, bulk_done, tb ? tb->a : nullptr)));
+      SOV_HIP_TRY((gemm_update_shape<T, false, true>(syrk_shape, rest, rest, nbA + nbB, l3, lda, l3, lda, a22, lda,
+                                                     T(-1), side_, bulk_done, tb ? tb->a : nullptr)));
       if (tb) {
         tb->launches = 1;
-        tb->flops = static_cast<double>(rest) * (rest + 1) * nb;
+        tb->flops = static_cast<double>(rest) * (rest + 1) * (nbA + nbB);
       }
     } else {
       SOV_HIP_TRY(hipEventRecord(ev_bulk_, side_));
       bulk_done = ev_bulk_;
     }
-    copy_panel_kernel<T><<<blocks_for(static_cast<long long>(below) * nb, 256), 256, 0, side_>>>(L21, a, lda, k0 + nb,
-                                                                                                  k0, below, nb);
-    SOV_LAUNCH_CHECK();
   }
   SOV_HIP_TRY(hipEventRecord(ev_bulk_, side_));
   SOV_HIP_TRY(hipStreamWaitEvent(s, ev_bulk_, 0));
