@@ -137,62 +137,79 @@ struct Prec<double> {
   __device__ static __forceinline__ double a(const Lp& L, long long idx) { return L.A[idx]; }
 };
 
-// out_j = (A^T vec)_j for j < n; vec (m) is staged in LDS first. One thread
-// per column over the row-major FP64 copy (consecutive threads read
-// consecutive columns of a row), four partial sums over rows i mod 4.
+// The two products below are bound by load round trips, not bytes: one
+// workgroup streams A (m x n FP64) through its CU, and a load of A takes
+// hundreds of cycles to return. Each thread therefore issues the loads of
+// a whole chunk (a column's rows, or a run of a row's columns) before it
+// uses any of them, so that one round trip serves many loads.
+
+// out_j = (A^T vec)_j for j < n, over the column-major A: 16 lanes per
+// column (rows r, r + 16, ...); vec (m) is staged in LDS first.
+constexpr int kDotLanes = 16;
+constexpr int kDotGroups = kThreads / kDotLanes;
+constexpr int kDotRows = (kMaxRows + kDotLanes - 1) / kDotLanes;  // rows of a column per lane
 __device__ void gemv_t(const Lp& L, const double* vec, double* out, double* stage) {
   const int tid = threadIdx.x;
   __syncthreads();
   for (int i = tid; i < L.m; i += kThreads) stage[i] = vec[i];
   __syncthreads();
-  const int m = L.m, ldr = L.ldr;
-  for (int j = tid; j < L.n; j += kThreads) {
-    const double* col = L.Ar + j;
-    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
-    int i = 0;
-    for (; i + 4 <= m; i += 4) {
-      s0 = fma(col[static_cast<long long>(i) * ldr], stage[i], s0);
-      s1 = fma(col[static_cast<long long>(i + 1) * ldr], stage[i + 1], s1);
-      s2 = fma(col[static_cast<long long>(i + 2) * ldr], stage[i + 2], s2);
-      s3 = fma(col[static_cast<long long>(i + 3) * ldr], stage[i + 3], s3);
+  const int m = L.m, n = L.n;
+  const int g = tid / kDotLanes, r = tid % kDotLanes;
+  for (int j = g; j < n; j += kDotGroups) {  // uniform within a group of 16 lanes
+    const double* c1 = L.A + static_cast<long long>(j) * m;
+    double a1[kDotRows];
+#pragma unroll
+    for (int q = 0; q < kDotRows; ++q) {
+      const int i = r + kDotLanes * q;
+      a1[q] = c1[i < m ? i : 0];
     }
-    for (; i < m; ++i) s0 = fma(col[static_cast<long long>(i) * ldr], stage[i], s0);
-    out[j] = (s0 + s1) + (s2 + s3);
+    double s1 = 0.0;
+#pragma unroll
+    for (int q = 0; q < kDotRows; ++q) {
+      const int i = r + kDotLanes * q;
+      s1 = fma(a1[q], i < m ? stage[i] : 0.0, s1);
+    }
+    s1 = group_sum<kDotLanes>(s1);
+    if (r == 0) out[j] = s1;
   }
   __syncthreads();
 }
 
-// out_i = base_i + (A vec)_i for i < m (base may be null). vec (n) global.
-// Threads take rows (consecutive threads read consecutive rows of a column),
-// kThreads / 256 slices of the columns, four partial sums per thread.
+// out_i = base_i + (A vec)_i for i < m (base may be null), over the
+// row-major FP64 copy of A: 16 lanes per row (columns r, r + 16, ...),
+// kChunkCols of them a load round. vec (n, global) is staged in LDS first
+// when it fits there.
+constexpr int kChunkCols = 12;
 __device__ void gemv_n(const Lp& L, const double* vec, const double* base, double* out, double* stage) {
-  constexpr int kSlices = kThreads / 256;  // slices of the columns, 256 rows each
   const int tid = threadIdx.x;
-  const int row = tid & 255, h = tid >> 8;
-  const int m = L.m, n = L.n;
-  for (int r0 = 0; r0 < m; r0 += 256) {
-    const int i = r0 + row;
-    double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
-    if (i < m) {
-      const double* ai = L.A + i;
-      int j = h;
-      for (; j + 3 * kSlices < n; j += 4 * kSlices) {
-        a0 = fma(ai[static_cast<long long>(j) * m], vec[j], a0);
-        a1 = fma(ai[static_cast<long long>(j + kSlices) * m], vec[j + kSlices], a1);
-        a2 = fma(ai[static_cast<long long>(j + 2 * kSlices) * m], vec[j + 2 * kSlices], a2);
-        a3 = fma(ai[static_cast<long long>(j + 3 * kSlices) * m], vec[j + 3 * kSlices], a3);
-      }
-      for (; j < n; j += kSlices) a0 = fma(ai[static_cast<long long>(j) * m], vec[j], a0);
-    }
+  const int m = L.m, n = L.n, ldr = L.ldr;
+  const double* v = vec;
+  if (n <= lay::kStage) {
     __syncthreads();
-    stage[h * 256 + row] = (a0 + a1) + (a2 + a3);
+    for (int j = tid; j < n; j += kThreads) stage[j] = vec[j];
     __syncthreads();
-    if (h == 0 && i < m) {
-      double s = stage[row];
+    v = stage;
+  }
+  const int g = tid / kDotLanes, r = tid % kDotLanes;
+  for (int i = g; i < m; i += kDotGroups) {  // uniform within a group of 16 lanes
+    const double* row = L.Ar + static_cast<long long>(i) * ldr;
+    double s0 = 0.0, s1 = 0.0;
+    for (int j0 = r; j0 < n; j0 += kDotLanes * kChunkCols) {
+      double av[kChunkCols];
 #pragma unroll
-      for (int q = 1; q < kSlices; ++q) s += stage[q * 256 + row];
-      out[i] = (base ? base[i] : 0.0) + s;
+      for (int u = 0; u < kChunkCols; ++u) {
+        const int j = j0 + kDotLanes * u;
+        av[u] = row[j < n ? j : 0];
+      }
+#pragma unroll
+      for (int u = 0; u < kChunkCols; u += 2) {
+        const int j = j0 + kDotLanes * u;
+        s0 = fma(av[u], j < n ? v[j] : 0.0, s0);
+        s1 = fma(av[u + 1], j + kDotLanes < n ? v[j + kDotLanes] : 0.0, s1);
+      }
     }
+    const double t = group_sum<kDotLanes>(s0 + s1);
+    if (r == 0) out[i] = (base ? base[i] : 0.0) + t;
   }
   __syncthreads();
 }
@@ -680,11 +697,63 @@ __device__ void invert_lower_blocked(const Lp& L, float* M) {
 }
 
 // q = A D A^T p (FP64, matrix-free); p, q global (m).
+// One pass over the column-major A: for each of its columns j a group of 16
+// lanes forms t_j = A(:, j)' p (p staged in LDS), then adds A(:, j) D_j t_j
+// into its rows of q, held in registers across the group's columns. The
+// four groups of a wavefront then combine by shuffles, and the wavefronts
+// add into LDS one after another, so the sum order is fixed.
+static_assert(2 * kMaxRows <= kStage, "normal_matvec stages p and q");
 __device__ void normal_matvec(const Lp& L, const double* p, double* q, double* stage) {
-  gemv_t(L, p, L.t, stage);
-  for (int j = threadIdx.x; j < L.n; j += kThreads) L.t[j] *= L.D[j];
+  const int tid = threadIdx.x, m = L.m, n = L.n;
+  double* pv = stage;             // p (m)
+  double* qs = stage + kMaxRows;  // q being summed (m)
   __syncthreads();
-  gemv_n(L, L.t, nullptr, q, stage);
+  for (int i = tid; i < m; i += kThreads) {
+    pv[i] = p[i];
+    qs[i] = 0.0;
+  }
+  __syncthreads();
+  const int g = tid / kDotLanes, r = tid % kDotLanes;
+  double qa[kDotRows];
+#pragma unroll
+  for (int k = 0; k < kDotRows; ++k) qa[k] = 0.0;
+  for (int j = g; j < n; j += kDotGroups) {  // uniform within a group of 16 lanes
+    const double* c1 = L.A + static_cast<long long>(j) * m;
+    double a1[kDotRows];
+#pragma unroll
+    for (int k = 0; k < kDotRows; ++k) {
+      const int i = r + kDotLanes * k;
+      a1[k] = c1[i < m ? i : 0];
+    }
+    const double dj = L.D[j];
+    double s1 = 0.0;
+#pragma unroll
+    for (int k = 0; k < kDotRows; ++k) {
+      const int i = r + kDotLanes * k;
+      s1 = fma(a1[k], i < m ? pv[i] : 0.0, s1);
+    }
+    const double f = group_sum<kDotLanes>(s1) * dj;
+#pragma unroll
+    for (int k = 0; k < kDotRows; ++k) qa[k] = fma(a1[k], f, qa[k]);
+  }
+#pragma unroll
+  for (int k = 0; k < kDotRows; ++k) {
+    qa[k] += __shfl_xor(qa[k], 16, 64);
+    qa[k] += __shfl_xor(qa[k], 32, 64);
+  }
+  const int lane = tid & 63, wave = tid >> 6;
+  for (int w = 0; w < kWaves; ++w) {
+    if (wave == w && lane < kDotLanes) {
+#pragma unroll
+      for (int k = 0; k < kDotRows; ++k) {
+        const int i = r + kDotLanes * k;
+        if (i < m) qs[i] += qa[k];
+      }
+    }
+    __syncthreads();
+  }
+  for (int i = tid; i < m; i += kThreads) q[i] = qs[i];
+  __syncthreads();
 }
 
 // dy = M^-1 rhs by PCG. Returns iterations.
