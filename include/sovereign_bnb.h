@@ -113,7 +113,9 @@ typedef struct {
 void sov_bnb_default_options(sov_bnb_options* opt);
 
 /* BranchAndBoundSolver::solve (branch_and_bound.cpp:906-1523) without
- * heuristics or node presolve, with cuts at the root only: up to
+ * primal heuristics or model presolve (a node presolve infeasibility test
+ * from the rows' activity ranges settles child LPs the interior point cannot
+ * certify infeasible), with cuts at the root only: up to
  * max_cut_rounds rounds of apply_cuts (:648-694, :1214-1273 at depth 0:
  * covers, Gomory rounding and, when asked, cMIR, each through the validity
  * gate, at most max_cuts_per_node a round, the root LP re-solved after each)

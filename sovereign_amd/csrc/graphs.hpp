@@ -11,7 +11,9 @@
 // Replay is opt-in (SOVEREIGN_GRAPHS=1): on MI355X it measured no faster
 // than launching the same kernels from the host (0.39 ms per FP64 solve pair
 // either way), and a run under rocprofv3's kernel tracing once crashed
-// inside the runtime's graph path. It is also off while per-kernel event
+// inside the runtime's graph path (cause not shown; graphs destroyed while
+// a replay could still be in flight were one defect found by reading, and
+// clear() now waits for their streams). It is also off while per-kernel event
 // timing is on (timing level 1), so that the event pairs see the individual
 // launches.
 #pragma once
@@ -44,8 +46,14 @@ class LaunchGraphs {
   LaunchGraphs& operator=(const LaunchGraphs&) = delete;
   ~LaunchGraphs() { clear(); }
 
+  // Destroys the cached graphs, after the streams they were last launched
+  // on have finished them (a replay may still be in flight when a caller
+  // reallocates the buffers the graphs hold, or tears the owner down).
   void clear() {
-    for (Entry& e : entries_) (void)hipGraphExecDestroy(e.exec);
+    for (Entry& e : entries_) {
+      if (e.stream) (void)hipStreamSynchronize(e.stream);
+      (void)hipGraphExecDestroy(e.exec);
+    }
     entries_.clear();
   }
 
@@ -56,14 +64,12 @@ class LaunchGraphs {
   template <typename F>
   int run(const void* k0, const void* k1, const void* k2, long long k3, hipStream_t s, F&& enqueue) {
     if (!graphs_enabled() || stream_capturing(s)) return enqueue();
-    for (const Entry& e : entries_)
-      if (e.k0 == k0 && e.k1 == k1 && e.k2 == k2 && e.k3 == k3)
+    for (Entry& e : entries_)
+      if (e.k0 == k0 && e.k1 == k1 && e.k2 == k2 && e.k3 == k3) {
+        e.stream = s;
         return hipGraphLaunch(e.exec, s) == hipSuccess ? SOV_OK : fail();
-    if (entries_.size() >= kMaxEntries) {
-      // the cached graphs may still be running on s
-      SOV_HIP_TRY(hipStreamSynchronize(s));
-      clear();
-    }
+      }
+    if (entries_.size() >= kMaxEntries) clear();
     SOV_HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
     const int rc = enqueue();
     hipGraph_t g = nullptr;
@@ -81,7 +87,7 @@ class LaunchGraphs {
       set_error(std::string("graph instantiation failed: ") + hipGetErrorString(ei));
       return SOV_ERR_DEVICE;
     }
-    entries_.push_back(Entry{k0, k1, k2, k3, exec});
+    entries_.push_back(Entry{k0, k1, k2, k3, exec, s});
     return hipGraphLaunch(exec, s) == hipSuccess ? SOV_OK : fail();
   }
 
@@ -93,6 +99,7 @@ class LaunchGraphs {
     const void* k2;
     long long k3;
     hipGraphExec_t exec;
+    hipStream_t stream;  // where it was last launched
   };
   static int fail() {
     set_error("graph launch failed");

@@ -10,9 +10,12 @@
 // one warm-started dual simplex after another.
 //
 // The tree search is the core of BranchAndBoundSolver::solve
-// (branch_and_bound.cpp:906-1523) without cuts, heuristics or presolve:
-// best-bound node selection, integrality check before pruning, and children
-// taken from the strong-branching solves of the chosen variable.
+// (branch_and_bound.cpp:906-1523) without primal heuristics or model
+// presolve: root cut rounds (cuts.hpp), best-bound node selection with
+// plunging, the reference's branching rules and pseudocost learning, a node
+// presolve infeasibility test, integrality checked before pruning, children
+// taken from the strong-branching solves of the chosen variable, and child
+// LPs warm-started from their parent's iterate (sovereign_bnb.h).
 
 #include "internal.hpp"
 #include "cuts.hpp"
