@@ -264,29 +264,44 @@ __device__ void assemble32(const Lp& L, float* M, double reg) {
       pb[c] = L.Af + static_cast<long long>(vb[c] ? cb : 0) * ldr + 4 * lk;
     }
     flt4_t acc[4] = {zero, zero, zero, zero};
-    for (int k0 = 0; k0 < ldr; k0 += 32) {  // ldr is a multiple of 32
-      const flt4_t d0 = *reinterpret_cast<const flt4_t*>(pd + k0);
-      const flt4_t d1 = *reinterpret_cast<const flt4_t*>(pd + k0 + 16);
-      flt4_t a0 = va ? *reinterpret_cast<const flt4_t*>(pa + k0) : zero;
-      flt4_t a1 = va ? *reinterpret_cast<const flt4_t*>(pa + k0 + 16) : zero;
-      flt4_t b0[4], b1[4];
+    // One 32-column chunk's operands in registers; the next chunk's loads
+    // are issued before this chunk's MFMAs, so their latency overlaps them.
+    flt4_t d0, d1, a0, a1, b0[4], b1[4];
+    auto load = [&](int k0) {
+      d0 = *reinterpret_cast<const flt4_t*>(pd + k0);
+      d1 = *reinterpret_cast<const flt4_t*>(pd + k0 + 16);
+      a0 = va ? *reinterpret_cast<const flt4_t*>(pa + k0) : zero;
+      a1 = va ? *reinterpret_cast<const flt4_t*>(pa + k0 + 16) : zero;
 #pragma unroll
       for (int c = 0; c < 4; ++c) {
         b0[c] = vb[c] ? *reinterpret_cast<const flt4_t*>(pb[c] + k0) : zero;
         b1[c] = vb[c] ? *reinterpret_cast<const flt4_t*>(pb[c] + k0 + 16) : zero;
       }
+    };
+    load(0);
+    for (int k0 = 0; k0 < ldr; k0 += 32) {  // ldr is a multiple of 32
+      flt4_t sa0 = a0, sa1 = a1, sb0[4], sb1[4];
 #pragma unroll
       for (int u = 0; u < 4; ++u) {
-        a0[u] *= d0[u];
-        a1[u] *= d1[u];
+        sa0[u] *= d0[u];
+        sa1[u] *= d1[u];
       }
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          sb0[c][u] = b0[c][u] * d0[u];
+          sb1[c][u] = b1[c][u] * d1[u];
+        }
+      }
+      if (k0 + 32 < ldr) load(k0 + 32);
 #pragma unroll
       for (int c = 0; c < 4; ++c) {
         if (c < nj) {
 #pragma unroll
-          for (int u = 0; u < 4; ++u) acc[c] = Prec<float>::mfma(a0[u], b0[c][u] * d0[u], acc[c]);
+          for (int u = 0; u < 4; ++u) acc[c] = Prec<float>::mfma(sa0[u], sb0[c][u], acc[c]);
 #pragma unroll
-          for (int u = 0; u < 4; ++u) acc[c] = Prec<float>::mfma(a1[u], b1[c][u] * d1[u], acc[c]);
+          for (int u = 0; u < 4; ++u) acc[c] = Prec<float>::mfma(sa1[u], sb1[c][u], acc[c]);
         }
       }
     }
