@@ -1107,10 +1107,10 @@ int DenseCholesky<T>::solve(const T* a, int m, double* x, double* /*work*/, hipS
   const int rc = solve_graphs_.run(a, x, nullptr, m, s, [&]() -> int {
     for (int P = 0; P < nsb; ++P) {  // L y = x: y into work_, x updated in place
       const int base = P * kSuper, sz = std::min(kSuper, m - base), below = m - base - sz;
-      gemv_n_kernel<T, false, true><<<blocks_for(sz, kGemvRows), 64 * kGemvWaves, 0, s>>>(
+      gemv_n_kernel<T, false, true><<<blocks_for(sz, kGemvRows), kGemvRows * kGemvPhases, 0, s>>>(
           sz, sz, sup_inv_ + P * sq, kSuper, x + base, work_ + base);
       if (below > 0)
-        gemv_n_kernel<T, true, false><<<blocks_for(below, kGemvRows), 64 * kGemvWaves, 0, s>>>(
+        gemv_n_kernel<T, true, false><<<blocks_for(below, kGemvRows), kGemvRows * kGemvPhases, 0, s>>>(
             below, sz, a + (base + sz) + base * lda, lda, work_ + base, x + base + sz);
     }
     SOV_LAUNCH_CHECK();
