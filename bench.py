@@ -171,6 +171,8 @@ def panel_roofline(kt, dtype, traffic):
             "frac": achieved / MFMA_PEAK[dtype], "traffic": tr["bytes_per_launch"] if tr else None,
             "kernel": name + " (diagonal block: factor + inverse, one workgroup, on the critical path)",
             "launches": int(launches), "avg_launch_us": ms * 1e3 / launches, "flops_per_launch": flops / launches,
+            "traffic_over_algorithmic_bytes": (tr["bytes_per_launch"] / tr["algorithmic_bytes_per_launch"]
+                                               if tr and tr.get("algorithmic_bytes_per_launch") else None),
             "traffic_source": tr.get("source") if tr else None}
 
 
