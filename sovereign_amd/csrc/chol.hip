@@ -819,7 +819,7 @@ int DenseCholesky<T>::reserve(int m) {
 // products (TRSM as a GEMM and the strip update, M x 128 x 128: 64x64,
 // 128x64, 128x128 or 32x32). On the first factorization in a precision each
 // shape runs a product of the class at a representative size (SYRK 4096 x
-// 4096 x 128; panel 4096 x 128 x 128, the middle of the factorization's
+// 4096 x 256, a panel pair; panel 4096 x 128 x 128, the middle of the factorization's
 // 8064 .. 128 rows), one warm-up and five timed launches (HIP events, the
 // fastest counts), and the fastest shape is kept for the process.
 // SOVEREIGN_GEMM_SHAPE = 0 / 1 / 2 / 3 forces one shape (128x64 falls back
@@ -875,9 +875,10 @@ int autotune_shapes(hipStream_t s) {
     return best;
   };
   int best_s = kGemm64x64, best_p = kGemm64x64;
+  // the bulk update applies a panel pair: K = 2 kPanel (a holds 4096 x 256)
   for (int shape : {kGemm64x64, kGemm128x128}) {
     t.syrk_ms[shape] = time_it([&] {
-      (void)gemm_update_shape<T, false, true>(shape, ms, ms, k, a, ms, a, ms, c, ms, T(-1), s);
+      (void)gemm_update_shape<T, false, true>(shape, ms, ms, 2 * k, a, ms, a, ms, c, ms, T(-1), s);
     });
     if (t.syrk_ms[shape] < t.syrk_ms[best_s]) best_s = shape;
   }
