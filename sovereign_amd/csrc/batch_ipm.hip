@@ -72,6 +72,7 @@ struct BatchOpts {
   double tol, tau, pcg_tol;
   int max_iter, pcg_max;
   double save_mu;  // the iterate written to state_out: the first with mu below this (<= 0: the final one)
+  long long lds_matrix_bytes;  // size of the LDS matrix region (lay::lds_matrix_bytes of the largest LP)
 };
 
 }  // namespace
@@ -447,6 +448,160 @@ __device__ void invert_lower(const Lp& L, T* M, double* tmp) {
     if (tid == 0) M[jj] = xjj;
     __syncthreads();
   }
+}
+
+// Blocked FP64 Cholesky of the packed lower matrix in global memory, for an
+// LP that has switched to FP64 factorizations: 16-column panels staged in
+// LDS (P: the rows k0..m-1 of the panel, row-major with leading dimension
+// 16, in the LDS region the FP32 matrix no longer needs), factored there
+// column by column with the pivot rule of cholesky() above, written back,
+// and the trailing lower triangle updated on FP64 MFMA from the panel (one
+// read-modify-write of the trailing matrix per 16 columns instead of one
+// per column). Returns the number of dropped pivots.
+__device__ int cholesky64_blocked(const Lp& L, double* M, double* P) {
+  constexpr int kB = lay::kPanel64;
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int lrow = lane & 15, lk = lane >> 4;
+  const int m = L.m;
+  int dropped = 0;
+  for (int k0 = 0; k0 < m; k0 += kB) {
+    const int kb = min(kB, m - k0), rows = m - k0;
+    __syncthreads();
+    for (int e = tid; e < rows * kB; e += kThreads) {
+      const int r = e / kB, c = e % kB;
+      P[e] = (c < kb && r >= c) ? M[packed_index(k0 + r, k0 + c, m)] : 0.0;
+    }
+    __syncthreads();
+    for (int j = 0; j < kb; ++j) {
+      const double dk = P[j * kB + j];
+      const bool drop = !(dk > 1e-13) && isfinite(dk);
+      const double piv = drop ? Prec<double>::drop_pivot : (isfinite(dk) ? sqrt(dk) : 1.0);
+      dropped += drop ? 1 : 0;
+      for (int r = j + 1 + tid; r < rows; r += kThreads) P[r * kB + j] /= piv;
+      __syncthreads();
+      if (tid == 0) P[j * kB + j] = piv;
+      for (int e = tid; e < rows * kB; e += kThreads) {
+        const int r = e / kB, c = e % kB;
+        if (c > j && c < kb && r >= c) P[e] -= P[r * kB + j] * P[c * kB + j];
+      }
+      __syncthreads();
+    }
+    for (int e = tid; e < rows * kB; e += kThreads) {
+      const int r = e / kB, c = e % kB;
+      if (c < kb && r >= c) M[packed_index(k0 + r, k0 + c, m)] = P[e];
+    }
+    // trailing update M(i, j) -= sum_c P(i, c) P(j, c), k0 + kb <= j <= i < m
+    const int tr = rows - kb;
+    const int nt = (tr + 15) / 16;
+    for (int t = wv; t < nt * (nt + 1) / 2; t += kWaves) {
+      int ti = 0;
+      while ((ti + 1) * (ti + 2) / 2 <= t) ++ti;
+      const int tj = t - ti * (ti + 1) / 2;
+      const int ra = kb + 16 * ti + lrow, cb = kb + 16 * tj + lrow;  // panel rows of the two operands
+      dbl4_t acc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+      for (int u = 0; u < kB / 4; ++u) {
+        const int kk = 4 * u + lk;
+        const double av = (ra < rows && kk < kb) ? P[ra * kB + kk] : 0.0;
+        const double bv = (cb < rows && kk < kb) ? P[cb * kB + kk] : 0.0;
+        acc = Prec<double>::mfma(av, bv, acc);
+      }
+      // lane holds column cb, rows kb + 16 ti + Prec<double>::row(lk, g)
+#pragma unroll
+      for (int g = 0; g < 4; ++g) {
+        const int i = k0 + kb + 16 * ti + Prec<double>::row(lk, g), j = k0 + cb;
+        if (i < m && j < m && i >= j) M[packed_index(i, j, m)] -= acc[g];
+      }
+    }
+  }
+  __syncthreads();
+  return dropped;
+}
+
+// In place: the packed FP64 factor L in global memory becomes X = L^-1,
+// block column by block column from the last (16 x 16 blocks):
+//   X(J, J) = L(J, J)^-1 (in LDS, one column of it per thread),
+//   T(I) = sum_{K = J+1 .. I} X(I, K) L(K, J) for every I > J, on FP64 MFMA,
+//   kept in LDS (T, 16 doubles per row below block J) until every L(., J)
+//   has been read, then X(I, J) = -T(I) X(J, J).
+// D and XJ: 16 x 16 LDS scratch each (the diagonal block and its inverse).
+__device__ void invert64_blocked(const Lp& L, double* M, double* T, double* D, double* XJ) {
+  constexpr int kB = lay::kPanel64;
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  const int lrow = lane & 15, lk = lane >> 4;
+  const int m = L.m, nbk = (m + kB - 1) / kB;
+  // element (i, k) of the block (I, K) of the lower triangle, zero outside it
+  auto at = [&](int i, int k) -> double { return (i < m && k < m && i >= k) ? M[packed_index(i, k, m)] : 0.0; };
+  for (int J = nbk - 1; J >= 0; --J) {
+    const int j0 = kB * J, jb = min(kB, m - j0);
+    __syncthreads();
+    for (int e = tid; e < kB * kB; e += kThreads) {
+      const int i = e / kB, k = e % kB;
+      D[e] = (i < jb && k < jb) ? at(j0 + i, j0 + k) : (i == k ? 1.0 : 0.0);
+    }
+    __syncthreads();
+    if (tid < kB) {  // column tid of the inverse of the lower triangular D
+      const int c = tid;
+      for (int i = 0; i < kB; ++i) {
+        double v;
+        if (i < c) {
+          v = 0.0;
+        } else {
+          double s = i == c ? 1.0 : 0.0;
+          for (int k = c; k < i; ++k) s -= D[i * kB + k] * XJ[k * kB + c];
+          v = s / D[i * kB + i];
+        }
+        XJ[i * kB + c] = v;
+      }
+    }
+    __syncthreads();
+    // T(I) for the block rows below J
+    const int nrow = nbk - 1 - J;
+    for (int q = wv; q < nrow; q += kWaves) {
+      const int I = J + 1 + q, i0 = kB * I;
+      dbl4_t acc = {0.0, 0.0, 0.0, 0.0};
+      for (int K = J + 1; K <= I; ++K) {
+        const int k0 = kB * K;
+        double av[kB / 4], bv[kB / 4];
+#pragma unroll
+        for (int u = 0; u < kB / 4; ++u) {
+          const int kk = 4 * u + lk;
+          av[u] = at(i0 + lrow, k0 + kk);  // X(I, K), zero above the diagonal of X(I, I)
+          bv[u] = at(k0 + kk, j0 + lrow);  // L(K, J)
+        }
+#pragma unroll
+        for (int u = 0; u < kB / 4; ++u) acc = Prec<double>::mfma(av[u], bv[u], acc);
+      }
+      // lane holds column lrow of T(I), rows Prec<double>::row(lk, g)
+#pragma unroll
+      for (int g = 0; g < 4; ++g) T[(q * kB + Prec<double>::row(lk, g)) * kB + lrow] = acc[g];
+    }
+    __syncthreads();
+    // X(I, J) = -T(I) X(J, J), and X(J, J) itself
+    for (int q = wv; q < nrow; q += kWaves) {
+      const int i0 = kB * (J + 1 + q);
+      double av[kB / 4], bv[kB / 4];
+#pragma unroll
+      for (int u = 0; u < kB / 4; ++u) {
+        const int kk = 4 * u + lk;
+        av[u] = T[(q * kB + lrow) * kB + kk];
+        bv[u] = XJ[kk * kB + lrow];
+      }
+      dbl4_t acc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+      for (int u = 0; u < kB / 4; ++u) acc = Prec<double>::mfma(av[u], bv[u], acc);
+#pragma unroll
+      for (int g = 0; g < 4; ++g) {
+        const int i = i0 + Prec<double>::row(lk, g), j = j0 + lrow;
+        if (i < m && lrow < jb) M[packed_index(i, j, m)] = -acc[g];
+      }
+    }
+    for (int e = tid; e < kB * kB; e += kThreads) {
+      const int i = e / kB, k = e % kB;
+      if (i < jb && k <= i) M[packed_index(j0 + i, j0 + k, m)] = XJ[e];
+    }
+  }
+  __syncthreads();
 }
 
 // vec <- M^-1 vec = S X^T X S vec (vec in LDS, FP64), with X = inverse of
@@ -1117,11 +1272,20 @@ __global__ __launch_bounds__(kThreads) void batch_ipm_kernel(const BatchMatrix* 
         else assemble32(L, M, 0.0);
         t0 = clock64();
         note(it, 10, static_cast<double>(t0 - t1));
-        note(it, 15, fp64 ? cholesky(L, L.M64) : cholesky_blocked(L, M, s_xkk));
+        // an LP in FP64 mode stages its panels in the LDS region the FP32
+        // matrix used, when they fit there
+        const bool blocked64 = lay::fp64_panel_fits(L.m, o.lds_matrix_bytes);
+        double* P64 = reinterpret_cast<double*>(M);
+        note(it, 15, fp64 ? (blocked64 ? cholesky64_blocked(L, L.M64, P64) : cholesky(L, L.M64))
+                          : cholesky_blocked(L, M, s_xkk));
         t1 = clock64();
         note(it, 11, static_cast<double>(t1 - t0));
-        if (fp64) invert_lower(L, L.M64, stage);
-        else invert_lower_blocked(L, M);
+        if (fp64) {
+          if (blocked64) invert64_blocked(L, L.M64, P64, stage, stage + lay::kPanel64 * lay::kPanel64);
+          else invert_lower(L, L.M64, stage);
+        } else {
+          invert_lower_blocked(L, M);
+        }
         // inexact Newton while mu is large: CG relative residual max(pcg_tol, min(1e-6, 1e-3 mu))
         const double ptol = fp64 ? o.pcg_tol : fmax(o.pcg_tol, fmin(1e-6, 1e-3 * mu));
         t0 = clock64();
@@ -1392,6 +1556,7 @@ int sov_batch_solve_warm(sov_batch* h, int32_t count, const int32_t* matrix_id, 
   o.pcg_tol = opt->refine_tol > 0 ? opt->refine_tol : 1e-12;
   o.pcg_max = opt->refine_max > 0 ? opt->refine_max : 30;
   o.save_mu = save_mu;
+  o.lds_matrix_bytes = lay::lds_matrix_bytes(mmax);
   int dev = 0, cus = 0;
   SOV_HIP_TRY(hipGetDevice(&dev));
   SOV_HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));

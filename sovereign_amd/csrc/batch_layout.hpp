@@ -59,6 +59,18 @@ SOV_HD constexpr std::size_t lds_off_matrix() { return sizeof(double) * (kStage 
 SOV_HD constexpr std::size_t lds_bytes(int mmax) {
   return lds_off_matrix() + sizeof(float) * static_cast<std::size_t>(mmax) * (mmax + 1) / 2;
 }
+// Bytes of the LDS matrix region (the packed FP32 normal matrix of the
+// launch's largest LP, mmax rows).
+SOV_HD constexpr long long lds_matrix_bytes(int mmax) {
+  return static_cast<long long>(sizeof(float)) * mmax * (mmax + 1) / 2;
+}
+// An LP that has switched to FP64 factorizations no longer uses that region:
+// its blocked FP64 Cholesky and inverse stage a 16-column panel of m rows
+// there (16 m doubles), when it fits.
+constexpr int kPanel64 = 16;
+SOV_HD constexpr bool fp64_panel_fits(int m, long long matrix_bytes) {
+  return static_cast<long long>(sizeof(double)) * kPanel64 * m <= matrix_bytes;
+}
 // Staging offsets (doubles) used by the preconditioner: vector and temporary.
 constexpr int kStagePv = 0, kStageTv = 512;
 static_assert(kStageTv >= kMaxRows && kStageTv + kMaxRows <= kStage, "preconditioner staging overlaps");

@@ -93,6 +93,28 @@ void audit_lp(int m, int n, std::uintptr_t wsp, int mmax) {
         in_range(a32 + 4 * (static_cast<std::uintptr_t>(i) * ldr + k), 16, a32, a32_end, 16, "rowsum A", m, n);
         in_range(sd + 4 * static_cast<std::uintptr_t>(k), 16, sd, sd_end, 16, "rowsum sd", m, n);
       }
+  // FP64 mode (an LP whose FP32 solves stalled): the blocked Cholesky stages
+  // a 16-column panel of up to m rows, and the blocked inverse the products
+  // T(I) of the block rows below J, in the LDS matrix region; the diagonal
+  // block and its inverse (2 x 256 doubles) in the staging area.
+  if (L::fp64_panel_fits(m, L::lds_matrix_bytes(mmax))) {
+    const std::uintptr_t reg = L::lds_off_matrix(), reg_end = L::lds_bytes(mmax);
+    ++checks;
+    if (static_cast<long long>(reg_end - reg) != L::lds_matrix_bytes(mmax)) fail("matrix region size", m, n, 0);
+    for (int k0 = 0; k0 < m; k0 += L::kPanel64) {
+      const int rows = m - k0;
+      in_range(reg, 8 * static_cast<std::size_t>(rows) * L::kPanel64, reg, reg_end, 8, "FP64 panel", m, n);
+    }
+    const int nbk = (m + L::kPanel64 - 1) / L::kPanel64;
+    for (int J = 0; J < nbk; ++J) {
+      const int nrow = nbk - 1 - J;
+      if (nrow > 0)
+        in_range(reg, 8 * static_cast<std::size_t>(nrow) * L::kPanel64 * L::kPanel64, reg, reg_end, 8, "FP64 T", m,
+                 n);
+    }
+    in_range(0, 8 * 2 * L::kPanel64 * L::kPanel64, 0, 8 * static_cast<std::uintptr_t>(L::kStage), 8, "FP64 D, XJ",
+             m, n);
+  }
   // tiles
   const int nt = (m + 15) / 16;
   const int items = L::assembly_items(nt);
