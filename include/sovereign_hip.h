@@ -311,15 +311,15 @@ int sov_lp_solve(const sov_lp_model* model, const sov_ipm_options* opt, sov_ipm_
  * + x_off[k] (n entries). status: 0 optimal, 5 iteration limit,
  * 6 numerical breakdown. The objective is c'x of the given data. Each LP's
  * normal matrix is factored in FP32 (LDS) as the preconditioner of FP64 CG;
- * an LP whose solves stall continues with FP64 factorizations in global
- * memory. */
+ * an LP whose FP32 factor drops a pivot or whose solves stall continues
+ * with FP64 factorizations in global memory. */
 typedef struct sov_batch sov_batch;
 typedef struct {
   int32_t status;
   int32_t iterations;
   int32_t pcg_iterations;
-  int32_t fp64_from; /* iteration from which the LP's factorizations were FP64 (its FP32-preconditioned
-                      * solves had stalled), or -1 */
+  int32_t fp64_from; /* iteration from which the LP's factorizations were FP64 (its FP32 factor had
+                      * dropped a pivot or its FP32-preconditioned solves had stalled), or -1 */
   double objective;
   double duality_gap;
   double primal_residual;

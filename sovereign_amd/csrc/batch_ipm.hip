@@ -20,7 +20,9 @@
 //                        (drop a pivot <= 1e-13 |diagonal|);
 //   M dy = rhs           FP64 preconditioned CG with the FP32 factor as the
 //                        preconditioner and the FP64 product A (D (A^T p)).
-// Residuals, step lengths and the termination test are FP64.
+// Residuals, step lengths and the termination test are FP64. An LP whose
+// FP32 factor drops a pivot, or whose CG solves stall, continues with FP64
+// factorizations of its normal matrix (packed in global memory, FP64 MFMA).
 
 #include "batch_layout.hpp"
 #include "internal.hpp"
@@ -109,7 +111,7 @@ struct Lp {
   // workspace vectors (global, this LP only)
   double *x, *w, *z, *v, *rc, *ru, *D, *dx, *dw, *dz, *dv, *dxa, *dwa, *dza, *dva, *t;
   double *y, *rb, *dy, *r, *p, *q, *zz, *rhs, *sc;  // sc: Jacobi scale of A D A^T
-  double* M64;  // packed lower FP64 normal matrix (global), for LPs whose FP32 solves stall
+  double* M64;  // packed lower FP64 normal matrix (global), for LPs in FP64 mode
   float* xs;    // inverses of the FP32 factor's 16 x 16 diagonal blocks (global, 256 floats each)
   float* sd;    // sqrt(D) in FP32, ldr entries (zero beyond n)
 };
@@ -135,7 +137,6 @@ struct Prec<double> {
   }
   typedef dbl4_t acc_t;
   __device__ static __forceinline__ int row(int lk, int g) { return lk + 4 * g; }
-  __device__ static __forceinline__ double a(const Lp& L, long long idx) { return L.A[idx]; }
 };
 
 // The two products below are bound by load round trips, not bytes: one
@@ -326,64 +327,85 @@ __device__ void assemble32(const Lp& L, float* M, double reg) {
   __syncthreads();
 }
 
-// Jacobi-scaled normal matrix (packed lower; the FP64 copy in global memory
-// for an LP whose FP32-preconditioned solves stalled):
+// FP64 Jacobi-scaled normal matrix (packed lower, in global memory) of an LP
+// that has switched to FP64 factorizations:
 //   Mt = S A D A^T S,  S = diag(sc),  sc_i = (A D A^T)_ii^(-1/2),
-// assembled on MFMA from the operands A(i, k) sqrt(D_k) sc_i, so Mt has a
-// unit diagonal whatever the spread of D; the FP32 factorization of Mt is
-// then accurate relative to its own scale. L.t holds sqrt(D) meanwhile.
-template <typename T>
-__device__ void assemble(const Lp& L, T* M, double reg) {
+// so Mt has a unit diagonal whatever the spread of D. Same work items as the
+// FP32 assembly above, over the row-major FP64 copy of A (zero beyond column
+// n): a lane's k-slot lk of MFMA step u is column k0 + 4 lk + u, both
+// operands use that map, and A(i, k) D_k A(j, k) is formed from the operands
+// A(i, k) D_k and A(j, k); S is applied to the FP64 sums when they are stored.
+__device__ void assemble64(const Lp& L, double* M, double reg) {
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   const int lrow = lane & 15, lk = lane >> 4;
-  for (int j = tid; j < L.n; j += kThreads) L.t[j] = sqrt(fmax(L.D[j], 0.0));
-  __syncthreads();
-  {  // diagonal in FP64: four threads per row
-    const int row = tid >> 2, part = tid & 3;
-    for (int i0 = 0; i0 < L.m; i0 += kThreads / 4) {
-      const int i = i0 + row;
-      double d = 0.0;
-      if (i < L.m)
-        for (int k = part; k < L.n; k += 4) {
-          const double a = L.A[i + static_cast<long long>(k) * L.m];
-          d = fma(a * a, L.D[k], d);
-        }
-      d += __shfl_xor(d, 1);
-      d += __shfl_xor(d, 2);
-      if (part == 0 && i < L.m) L.sc[i] = d + reg > 0.0 ? 1.0 / sqrt(d + reg) : 1.0;
+  const int m = L.m, n = L.n, ldr = L.ldr;
+  for (int i = wv; i < m; i += kWaves) {  // diagonal: a wavefront per row
+    const double* ar = L.Ar + static_cast<long long>(i) * ldr;
+    double acc = 0.0;
+    for (int k = lane; k < n; k += 64) {
+      const double a = ar[k];
+      acc = fma(a * a, fmax(L.D[k], 0.0), acc);
     }
+    const double dii = wave_sum(acc);
+    if (lane == 0) L.sc[i] = dii + reg > 0.0 ? 1.0 / sqrt(dii + reg) : 1.0;
   }
   __syncthreads();
-  const int nt = (L.m + 15) / 16;
-  const int tiles = nt * (nt + 1) / 2;
-  for (int t = wv; t < tiles; t += kWaves) {
-    int ti = static_cast<int>((sqrt(8.0 * t + 1.0) - 1.0) * 0.5);
-    while ((ti + 1) * (ti + 2) / 2 <= t) ++ti;
-    while (ti * (ti + 1) / 2 > t) --ti;
-    const int tj = t - ti * (ti + 1) / 2;
-    const int r0 = ti * 16, c0 = tj * 16;
-    const int ra = r0 + lrow, cb = c0 + lrow;
-    const double sa = ra < L.m ? L.sc[ra] : 0.0, sbv = cb < L.m ? L.sc[cb] : 0.0;
-    typename Prec<T>::acc_t acc = {0, 0, 0, 0};
-    for (int k0 = 0; k0 < L.n; k0 += 16) {
-      T av[4], bv[4];
+  const int nt = (m + 15) / 16;
+  const int nitems = lay::assembly_items(nt);
+  const dbl4_t zero = {0.0, 0.0, 0.0, 0.0};
+  for (int q = wv; q < nitems; q += kWaves) {
+    int I, J0;
+    lay::assembly_item(q, I, J0);
+    const int nj = min(4, I + 1 - J0);
+    const int ra = 16 * I + lrow;
+    const bool va = ra < m;
+    const double* pa = L.Ar + static_cast<long long>(va ? ra : 0) * ldr + 4 * lk;
+    const double* pb[4];
+    bool vb[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      const int cb = 16 * (J0 + c) + lrow;
+      vb[c] = c < nj && cb < m;
+      pb[c] = L.Ar + static_cast<long long>(vb[c] ? cb : 0) * ldr + 4 * lk;
+    }
+    dbl4_t acc[4] = {zero, zero, zero, zero};
+    // One 16-column step's operands in registers; the next step's loads are
+    // issued before this step's MFMAs.
+    dbl4_t a, d, b[4];
+    auto load = [&](int k0) {
+      a = va ? *reinterpret_cast<const dbl4_t*>(pa + k0) : zero;
 #pragma unroll
       for (int u = 0; u < 4; ++u) {
-        const int k = k0 + 4 * u + lk;
-        const bool kin = k < L.n;
-        const double sd = kin ? L.t[k] : 0.0;
-        av[u] = (kin && ra < L.m) ? Prec<T>::a(L, ra + static_cast<long long>(k) * L.m) * static_cast<T>(sd * sa) : T(0);
-        bv[u] = (kin && cb < L.m) ? Prec<T>::a(L, cb + static_cast<long long>(k) * L.m) * static_cast<T>(sd * sbv) : T(0);
+        const int k = k0 + 4 * lk + u;
+        d[u] = k < n ? fmax(L.D[k], 0.0) : 0.0;
       }
 #pragma unroll
-      for (int u = 0; u < 4; ++u) acc = Prec<T>::mfma(av[u], bv[u], acc);
-    }
-    // lane holds column c0 + lrow, rows r0 + Prec<T>::row(lk, g)
+      for (int c = 0; c < 4; ++c) b[c] = vb[c] ? *reinterpret_cast<const dbl4_t*>(pb[c] + k0) : zero;
+    };
+    load(0);
+    for (int k0 = 0; k0 < ldr; k0 += 16) {  // ldr is a multiple of 32
+      dbl4_t sa = a * d, sb[4];
 #pragma unroll
-    for (int g = 0; g < 4; ++g) {
-      const int i = r0 + Prec<T>::row(lk, g), j = c0 + lrow;
-      if (i < L.m && j < L.m && i >= j)
-        M[packed_index(i, j, L.m)] = acc[g] + (i == j ? static_cast<T>(reg * L.sc[i] * L.sc[i]) : T(0));
+      for (int c = 0; c < 4; ++c) sb[c] = b[c];
+      if (k0 + 16 < ldr) load(k0 + 16);
+#pragma unroll
+      for (int c = 0; c < 4; ++c)
+        if (c < nj) {
+#pragma unroll
+          for (int u = 0; u < 4; ++u) acc[c] = Prec<double>::mfma(sa[u], sb[c][u], acc[c]);
+        }
+    }
+    // lane holds column 16 (J0 + c) + lrow, rows 16 I + Prec<double>::row(lk, g)
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      const int j = 16 * (J0 + c) + lrow;
+      if (c >= nj || j >= m) continue;
+      const double sj = L.sc[j];
+#pragma unroll
+      for (int g = 0; g < 4; ++g) {
+        const int i = 16 * I + Prec<double>::row(lk, g);
+        if (i < m && i >= j) M[packed_index(i, j, m)] = L.sc[i] * sj * (acc[c][g] + (i == j ? reg : 0.0));
+      }
     }
   }
   __syncthreads();
@@ -1062,6 +1084,7 @@ __global__ __launch_bounds__(kThreads) void batch_ipm_kernel(const BatchMatrix* 
   double* red = reinterpret_cast<double*>(smem + lay::lds_off_red());  // kRed doubles
   float* M = reinterpret_cast<float*>(smem + lay::lds_off_matrix());   // packed lower triangle
   __shared__ int s_lp;
+  __shared__ int s_drop;  // pivots the FP32 factorization dropped
   __shared__ Lp s_L;
   __shared__ float s_xkk[kBlk * (kBlk + 1)];  // inverse of the diagonal block being factored
   const int tid = threadIdx.x;
@@ -1149,7 +1172,7 @@ __global__ __launch_bounds__(kThreads) void batch_ipm_kernel(const BatchMatrix* 
       if (tr && tid == 0 && it < kTraceRows) tr[it * kTraceCols + col] = v;
     };
     int stalled = 0;  // consecutive FP64 iterations whose two PCG solves both hit the cap
-    bool fp64 = false;  // FP64 factorizations (global memory) after the FP32 solves stalled
+    bool fp64 = false;  // FP64 factorizations (global memory): the FP32 factor failed
     int fp64_from = -1;
     double best = 1e300;
     int best_it = 0;
@@ -1268,7 +1291,7 @@ __global__ __launch_bounds__(kThreads) void batch_ipm_kernel(const BatchMatrix* 
       // iteration is redone (a step along the unconverged direction can undo
       // the primal feasibility of a nearly solved LP).
       for (;;) {
-        if (fp64) assemble(L, L.M64, 0.0);
+        if (fp64) assemble64(L, L.M64, 0.0);
         else assemble32(L, M, 0.0);
         t0 = clock64();
         note(it, 10, static_cast<double>(t0 - t1));
@@ -1276,10 +1299,25 @@ __global__ __launch_bounds__(kThreads) void batch_ipm_kernel(const BatchMatrix* 
         // matrix used, when they fit there
         const bool blocked64 = lay::fp64_panel_fits(L.m, o.lds_matrix_bytes);
         double* P64 = reinterpret_cast<double*>(M);
-        note(it, 15, fp64 ? (blocked64 ? cholesky64_blocked(L, L.M64, P64) : cholesky(L, L.M64))
-                          : cholesky_blocked(L, M, s_xkk));
+        const int dropped = fp64 ? (blocked64 ? cholesky64_blocked(L, L.M64, P64) : cholesky(L, L.M64))
+                                 : cholesky_blocked(L, M, s_xkk);
+        note(it, 15, dropped);
         t1 = clock64();
         note(it, 11, static_cast<double>(t1 - t0));
+        // An FP32 factor that dropped a pivot preconditions the solves badly
+        // (they then take tens of CG steps, each dearer than the FP64
+        // factorization): the LP switches to FP64 and redoes the iteration.
+        if (!fp64) {
+          if (tid == 0) s_drop = dropped;  // counted on wavefront 0
+          __syncthreads();
+          if (s_drop > 0) {
+            fp64 = true;
+            fp64_from = it;
+            stalled = 0;
+            best_it = it;
+            continue;
+          }
+        }
         if (fp64) {
           if (blocked64) invert64_blocked(L, L.M64, P64, stage, stage + lay::kPanel64 * lay::kPanel64);
           else invert_lower(L, L.M64, stage);

@@ -10,7 +10,8 @@
 //   * every 16-byte load of the FP32 assembly (row sums and tiles, every lane
 //     of every work item, whether its predicate is on or off) lies inside the
 //     row-major FP32 copy of A (m x ldr floats) or inside sqrt(D), 16-byte
-//     aligned;
+//     aligned, and every 32-byte load of the FP64 assembly's tiles inside the
+//     row-major FP64 copy (m x ldr doubles), 32-byte aligned;
 //   * the work items cover every lower 16 x 16 tile exactly once, and every
 //     packed-matrix store lies inside the LDS bytes the launch reserves;
 //   * the dynamic LDS of the largest LP, with room for the kernel's static
@@ -31,8 +32,9 @@ long long checks = 0;
 int failures = 0;
 // Planted mistakes: with argument "w" each LP's workspace ends 16 bytes
 // early (the spare double and the alignment slack), with "a" the FP32 copy
-// of A ends one float early; the audit must then fail.
-std::uintptr_t plant_ws = 0, plant_a32 = 0;
+// of A ends one float early, with "d" the FP64 copy one double early; the
+// audit must then fail.
+std::uintptr_t plant_ws = 0, plant_a32 = 0, plant_a64 = 0;
 
 void fail(const char* what, int m, int n, long long detail) {
   if (failures < 20) std::printf("FAIL %s m=%d n=%d detail=%lld\n", what, m, n, detail);
@@ -86,6 +88,8 @@ void audit_lp(int m, int n, std::uintptr_t wsp, int mmax) {
   const std::uintptr_t a32 = 0x40000000;
   const std::uintptr_t a32_end = a32 + 4 * static_cast<std::uintptr_t>(m) * ldr - plant_a32;
   const std::uintptr_t sd_end = sd + 4 * static_cast<std::uintptr_t>(ldr);
+  const std::uintptr_t a64 = 0x80000000;
+  const std::uintptr_t a64_end = a64 + 8 * static_cast<std::uintptr_t>(m) * ldr - plant_a64;
   // row sums: wavefront per row, lane takes k = 4 lane + 256 t
   for (int i = 0; i < m; ++i)
     for (int lane = 0; lane < 64; ++lane)
@@ -148,6 +152,18 @@ void audit_lp(int m, int n, std::uintptr_t wsp, int mmax) {
           in_range(pa + off, 16, a32, a32_end, 16, "tile A row", m, n);
           for (int c = 0; c < 4; ++c) in_range(pb[c] + off, 16, a32, a32_end, 16, "tile A column", m, n);
         }
+      // FP64 mode: 32-byte loads of the row-major FP64 copy of A (m x ldr
+      // doubles, its own 256-byte aligned allocation), 16 columns a step
+      const std::uintptr_t pa64 = a64 + 8 * (static_cast<std::uintptr_t>(ra < m ? ra : 0) * ldr + 4 * lk);
+      for (int k0 = 0; k0 < ldr; k0 += 16) {
+        in_range(pa64 + 8 * static_cast<std::uintptr_t>(k0), 32, a64, a64_end, 32, "FP64 tile A row", m, n);
+        for (int c = 0; c < 4; ++c) {
+          const int cb = 16 * (J0 + c) + lrow;
+          const bool vb = c < nj && cb < m;
+          const std::uintptr_t p = a64 + 8 * (static_cast<std::uintptr_t>(vb ? cb : 0) * ldr + 4 * lk + k0);
+          in_range(p, 32, a64, a64_end, 32, "FP64 tile A column", m, n);
+        }
+      }
       // stores of the packed matrix: lane column 16 (J0 + c) + lrow, rows 16 I + 4 lk + g
       for (int c = 0; c < nj; ++c) {
         const int j = 16 * (J0 + c) + lrow;
@@ -175,6 +191,7 @@ void audit_lp(int m, int n, std::uintptr_t wsp, int mmax) {
 int main(int argc, char** argv) {
   if (argc > 1 && argv[1][0] == 'w') plant_ws = 8 + 8;  // past the spare double and the alignment slack
   if (argc > 1 && argv[1][0] == 'a') plant_a32 = 4;
+  if (argc > 1 && argv[1][0] == 'd') plant_a64 = 8;
   // static LDS of the kernel (queue slot, LP descriptor, a 16 x 17 float block) is
   // under 2 KiB; the dynamic part of the largest LP must leave room for it
   ++checks;

@@ -2,9 +2,10 @@
 host: tests/batch_layout_audit.cpp, compiled with g++ against the layout
 header the kernel and its host code use (sovereign_amd/csrc/
 batch_layout.hpp), walks every workspace region and every 16-byte load of
-the FP32 assembly for all row counts 1..272 against the allocation ends. Two
-planted mistakes (a workspace 16 bytes short, an A copy one float short)
-must make it fail."""
+the FP32 assembly and every 32-byte load of the FP64 assembly for all row
+counts 1..272 against the allocation ends. Three planted mistakes (a
+workspace 16 bytes short, the FP32 copy of A one float short, the FP64 copy
+one double short) must make it fail."""
 import os
 import shutil
 import subprocess
@@ -32,7 +33,7 @@ def test_batch_layout_in_bounds(audit):
     assert r.stdout.startswith("ok ")
 
 
-@pytest.mark.parametrize("plant", ["w", "a"])
+@pytest.mark.parametrize("plant", ["w", "a", "d"])
 def test_batch_layout_audit_catches_planted_mistakes(audit, plant):
     r = subprocess.run([audit, plant], capture_output=True, text=True, timeout=120)
     assert r.returncode == 1 and "FAIL" in r.stdout
