@@ -948,6 +948,64 @@ __device__ void normal_matvec(const Lp& L, const double* p, double* q, double* s
   __syncthreads();
 }
 
+// The residual products in one pass over the column-major A, as in
+// normal_matvec: rb = b - A x (m) and t = A^T y (n). Each 16-lane group
+// forms t_j = A(:, j)'y (y staged in LDS) and adds A(:, j) x_j into its rows
+// of A x, held in registers; the wavefronts add into LDS in a fixed order.
+__device__ void residual_products(const Lp& L, double* stage) {
+  const int tid = threadIdx.x, m = L.m, n = L.n;
+  double* yv = stage;             // y (m)
+  double* qs = stage + kMaxRows;  // A x being summed (m)
+  __syncthreads();
+  for (int i = tid; i < m; i += kThreads) {
+    yv[i] = L.y[i];
+    qs[i] = 0.0;
+  }
+  __syncthreads();
+  const int g = tid / kDotLanes, r = tid % kDotLanes;
+  double qa[kDotRows];
+#pragma unroll
+  for (int k = 0; k < kDotRows; ++k) qa[k] = 0.0;
+  for (int j = g; j < n; j += kDotGroups) {  // uniform within a group of 16 lanes
+    const double* c1 = L.A + static_cast<long long>(j) * m;
+    double a1[kDotRows];
+#pragma unroll
+    for (int k = 0; k < kDotRows; ++k) {
+      const int i = r + kDotLanes * k;
+      a1[k] = c1[i < m ? i : 0];
+    }
+    const double xj = L.x[j];
+    double s1 = 0.0;
+#pragma unroll
+    for (int k = 0; k < kDotRows; ++k) {
+      const int i = r + kDotLanes * k;
+      s1 = fma(a1[k], i < m ? yv[i] : 0.0, s1);
+    }
+    const double tj = group_sum<kDotLanes>(s1);
+    if (r == 0) L.t[j] = tj;
+#pragma unroll
+    for (int k = 0; k < kDotRows; ++k) qa[k] = fma(a1[k], xj, qa[k]);
+  }
+#pragma unroll
+  for (int k = 0; k < kDotRows; ++k) {
+    qa[k] += __shfl_xor(qa[k], 16, 64);
+    qa[k] += __shfl_xor(qa[k], 32, 64);
+  }
+  const int lane = tid & 63, wave = tid >> 6;
+  for (int w = 0; w < kWaves; ++w) {
+    if (wave == w && lane < kDotLanes) {
+#pragma unroll
+      for (int k = 0; k < kDotRows; ++k) {
+        const int i = r + kDotLanes * k;
+        if (i < m) qs[i] += qa[k];
+      }
+    }
+    __syncthreads();
+  }
+  for (int i = tid; i < m; i += kThreads) L.rb[i] = L.b[i] - qs[i];
+  __syncthreads();
+}
+
 // dy = M^-1 rhs by PCG. Returns iterations.
 template <typename T>
 __device__ int solve_normal(const Lp& L, const T* M, double* stage, double* red, double tol, int max_it) {
@@ -1180,14 +1238,11 @@ __global__ __launch_bounds__(kThreads) void batch_ipm_kernel(const BatchMatrix* 
     for (int it = 0; it < o.max_iter; ++it) {
       __syncthreads();
       long long t0 = clock64();
-      // residuals
-      gemv_n(L, L.x, nullptr, L.rb, stage);  // rb = A x (then b - A x)
-      gemv_t(L, L.y, L.t, stage);            // t = A^T y
+      // residuals: rb = b - A x, t = A^T y
+      residual_products(L, stage);
       double rbmax = 0.0, rcmax = 0.0, rumax = 0.0, cx = 0.0, by = 0.0, uv = 0.0, xz = 0.0;
       for (int i = tid; i < m; i += kThreads) {
-        const double r = L.b[i] - L.rb[i];
-        L.rb[i] = r;
-        rbmax = fmax(rbmax, fabs(r));
+        rbmax = fmax(rbmax, fabs(L.rb[i]));
         by += L.b[i] * L.y[i];
       }
       for (int j = tid; j < n; j += kThreads) {
