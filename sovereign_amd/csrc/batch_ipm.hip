@@ -888,14 +888,15 @@ __device__ void invert_lower_blocked(const Lp& L, float* M) {
   }
 }
 
-// q = A D A^T p (FP64, matrix-free); p, q global (m).
+// q = A D A^T p (FP64, matrix-free); p, q global (m); atp (n, global, or
+// null) receives A^T p.
 // One pass over the column-major A: for each of its columns j a group of 16
 // lanes forms t_j = A(:, j)' p (p staged in LDS), then adds A(:, j) D_j t_j
 // into its rows of q, held in registers across the group's columns. The
 // four groups of a wavefront then combine by shuffles, and the wavefronts
 // add into LDS one after another, so the sum order is fixed.
 static_assert(2 * kMaxRows <= kStage, "normal_matvec stages p and q");
-__device__ void normal_matvec(const Lp& L, const double* p, double* q, double* stage) {
+__device__ void normal_matvec(const Lp& L, const double* p, double* q, double* stage, double* atp) {
   const int tid = threadIdx.x, m = L.m, n = L.n;
   double* pv = stage;             // p (m)
   double* qs = stage + kMaxRows;  // q being summed (m)
@@ -924,7 +925,9 @@ __device__ void normal_matvec(const Lp& L, const double* p, double* q, double* s
       const int i = r + kDotLanes * k;
       s1 = fma(a1[k], i < m ? pv[i] : 0.0, s1);
     }
-    const double f = group_sum<kDotLanes>(s1) * dj;
+    const double tj = group_sum<kDotLanes>(s1);
+    if (atp && r == 0) atp[j] = tj;
+    const double f = tj * dj;
 #pragma unroll
     for (int k = 0; k < kDotRows; ++k) qa[k] = fma(a1[k], f, qa[k]);
   }
@@ -1006,9 +1009,13 @@ __device__ void residual_products(const Lp& L, double* stage) {
   __syncthreads();
 }
 
-// dy = M^-1 rhs by PCG. Returns iterations.
+// dy = M^-1 rhs by PCG. Returns iterations. aty (n) receives A^T dy, kept
+// alongside dy from the products the iteration forms anyway: A^T x0 from the
+// first residual's product, then aty += alpha A^T p with A^T p (atp, n
+// scratch) from each step's product.
 template <typename T>
-__device__ int solve_normal(const Lp& L, const T* M, double* stage, double* red, double tol, int max_it) {
+__device__ int solve_normal(const Lp& L, const T* M, double* stage, double* red, double tol, int max_it, double* aty,
+                            double* atp) {
   const int tid = threadIdx.x, m = L.m;
   double* pv = stage + lay::kStagePv;  // m doubles: preconditioner vector (LDS)
   double* tv = stage + lay::kStageTv;  // m doubles: preconditioner temporary (LDS)
@@ -1017,7 +1024,7 @@ __device__ int solve_normal(const Lp& L, const T* M, double* stage, double* red,
   precondition(L, M, pv, tv);
   for (int i = tid; i < m; i += kThreads) L.dy[i] = pv[i];
   __syncthreads();
-  normal_matvec(L, L.dy, L.q, stage);
+  normal_matvec(L, L.dy, L.q, stage, aty);
   double rn = 0.0, bn = 0.0;
   for (int i = tid; i < m; i += kThreads) {
     const double r = L.rhs[i] - L.q[i];
@@ -1040,11 +1047,12 @@ __device__ int solve_normal(const Lp& L, const T* M, double* stage, double* red,
   int it = 0;
   for (it = 1; it <= max_it; ++it) {
     __syncthreads();
-    normal_matvec(L, L.p, L.q, stage);
+    normal_matvec(L, L.p, L.q, stage, atp);
     double pq = 0.0;
     for (int i = tid; i < m; i += kThreads) pq += L.p[i] * L.q[i];
     pq = bred<0>(pq, red);
     const double alpha = pq != 0.0 ? rz / pq : 0.0;
+    for (int j = tid; j < L.n; j += kThreads) aty[j] = fma(alpha, atp[j], aty[j]);
     rn = 0.0;
     for (int i = tid; i < m; i += kThreads) {
       L.dy[i] += alpha * L.p[i];
@@ -1087,17 +1095,20 @@ __device__ int newton(const Lp& L, const T* M, double* stage, double* red, const
   }
   __syncthreads();
   gemv_n(L, L.t, L.rb, L.rhs, stage);
-  // keep t: gemv_t / normal_matvec overwrite L.t, so stash it in dx first
+  // t moves to dx, where the last loop reads it element by element
   for (int j = tid; j < L.n; j += kThreads) dx[j] = L.t[j];
   __syncthreads();
-  const int its = solve_normal(L, M, stage, red, pcg_tol, o.pcg_max);
-  gemv_t(L, L.dy, L.t, stage);  // t = A^T dy
+  // A^T dy comes back in dw (dv: scratch), both written below only after
+  // their element has been read
+  const int its = solve_normal(L, M, stage, red, pcg_tol, o.pcg_max, dw, dv);
+  __syncthreads();
   for (int j = tid; j < L.n; j += kThreads) {
+    const double atdy = dw[j];
     if (!active(L, j)) {
       dx[j] = dw[j] = dz[j] = dv[j] = 0.0;
       continue;
     }
-    const double dxj = L.D[j] * L.t[j] - dx[j];
+    const double dxj = L.D[j] * atdy - dx[j];
     dx[j] = dxj;
     dz[j] = (r1[j] - L.z[j] * dxj) / L.x[j];
     if (bounded(L, j)) {
