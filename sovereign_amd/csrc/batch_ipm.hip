@@ -751,19 +751,21 @@ __device__ int leaf16(float* M, int k0, int kb, int m, float* sx, float* xg, int
 // diagonal block in registers (leaf16); (2) the panel below it is
 // L(I, K) = A(I, K) X(K, K)^T, one 16-row MFMA tile per wavefront; (3) the
 // trailing lower triangle takes the panel's rank-16 update on MFMA
-// (v_mfma_f32_16x16x4f32), one 16 x 16 tile per wavefront at a time. Three
-// barriers a block. The diagonal blocks' inverses go to xg (16 x 16 each,
-// row-major) for invert_lower_blocked. Returns the dropped pivots (counted
-// on wavefront 0).
+// (v_mfma_f32_16x16x4f32), one 16 x 16 tile per wavefront at a time, except
+// that wavefront 0 takes the next diagonal block's tile and then factors
+// that block (step (1) of the next block, overlapping the rest of the
+// update). Two barriers a block. The diagonal blocks' inverses go to xg
+// (16 x 16 each, row-major) for invert_lower_blocked. Returns the dropped
+// pivots (counted on wavefront 0).
 __device__ int cholesky_blocked(const Lp& L, float* M, float* sx) {
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   const int lrow = lane & 15, lk = lane >> 4;
   const int m = L.m;
   int dropped = 0;
+  if (wv == 0) dropped += leaf16(M, 0, min(kBlk, m), m, sx, L.xs, lane);
+  __syncthreads();
   for (int k0 = 0; k0 < m; k0 += kBlk) {
     const int kb = min(kBlk, m - k0);
-    if (wv == 0) dropped += leaf16(M, k0, kb, m, sx, L.xs + (k0 / kBlk) * kBlk * kBlk, lane);
-    __syncthreads();
     const int r0 = k0 + kb;  // first row below the block
     if (r0 >= m) break;      // (only the last block is narrower than kBlk)
     for (int pt = wv; r0 + kBlk * pt < m; pt += kWaves) {  // (2) rows r0 + 16 pt .. of the panel
@@ -784,31 +786,40 @@ __device__ int cholesky_blocked(const Lp& L, float* M, float* sx) {
       }
     }
     __syncthreads();
-    {  // (3) trailing update, tiles (I, J), J <= I, of the rows and columns from r0
-      const int nt = (m - r0 + kBlk - 1) / kBlk;
-      const int tiles = nt * (nt + 1) / 2;
-      for (int t = wv; t < tiles; t += kWaves) {
-        int ti = static_cast<int>((sqrt(8.0 * t + 1.0) - 1.0) * 0.5);
-        while ((ti + 1) * (ti + 2) / 2 <= t) ++ti;
-        while (ti * (ti + 1) / 2 > t) --ti;
-        const int tj = t - ti * (ti + 1) / 2;
-        const int ra = r0 + kBlk * ti + lrow, cb = r0 + kBlk * tj + lrow;
-        float av[4], bv[4];
+    // (3) trailing update, tiles (I, J), J <= I, of the rows and columns from
+    // r0. Look-ahead: wavefront 0 updates tile (0, 0), the next diagonal
+    // block, and factors it at once (the leaf reads and writes that tile
+    // only) while the other wavefronts update the rest.
+    const int nt = (m - r0 + kBlk - 1) / kBlk;
+    const int tiles = nt * (nt + 1) / 2;
+    auto update_tile = [&](int t) {
+      int ti = static_cast<int>((sqrt(8.0 * t + 1.0) - 1.0) * 0.5);
+      while ((ti + 1) * (ti + 2) / 2 <= t) ++ti;
+      while (ti * (ti + 1) / 2 > t) --ti;
+      const int tj = t - ti * (ti + 1) / 2;
+      const int ra = r0 + kBlk * ti + lrow, cb = r0 + kBlk * tj + lrow;
+      float av[4], bv[4];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
-          const int k = k0 + 4 * u + lk;
-          av[u] = packed_at(M, ra, k, m);
-          bv[u] = packed_at(M, cb, k, m);
-        }
-        flt4_t acc = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-        for (int u = 0; u < 4; ++u) acc = Prec<float>::mfma(av[u], bv[u], acc);
-#pragma unroll
-        for (int g = 0; g < 4; ++g) {
-          const int i = r0 + kBlk * ti + Prec<float>::row(lk, g), j = cb;
-          if (i < m && j < m && i >= j) M[packed_index(i, j, m)] -= acc[g];
-        }
+      for (int u = 0; u < 4; ++u) {
+        const int k = k0 + 4 * u + lk;
+        av[u] = packed_at(M, ra, k, m);
+        bv[u] = packed_at(M, cb, k, m);
       }
+      flt4_t acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int u = 0; u < 4; ++u) acc = Prec<float>::mfma(av[u], bv[u], acc);
+#pragma unroll
+      for (int g = 0; g < 4; ++g) {
+        const int i = r0 + kBlk * ti + Prec<float>::row(lk, g), j = cb;
+        if (i < m && j < m && i >= j) M[packed_index(i, j, m)] -= acc[g];
+      }
+    };
+    if (wv == 0) {
+      update_tile(0);
+      wave_sync();
+      dropped += leaf16(M, r0, min(kBlk, m - r0), m, sx, L.xs + (r0 / kBlk) * kBlk * kBlk, lane);
+    } else {
+      for (int t = wv; t < tiles; t += kWaves - 1) update_tile(t);
     }
     __syncthreads();
   }
