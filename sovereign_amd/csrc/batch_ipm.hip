@@ -918,24 +918,27 @@ __device__ void normal_matvec(const Lp& L, const double* p, double* q, double* s
   }
   __syncthreads();
   const int g = tid / kDotLanes, r = tid % kDotLanes;
-  double qa[kDotRows];
+  double qa[kDotRows], pr[kDotRows];  // pr: the lane's rows of p, the same for every column
 #pragma unroll
-  for (int k = 0; k < kDotRows; ++k) qa[k] = 0.0;
+  for (int k = 0; k < kDotRows; ++k) {
+    const int i = r + kDotLanes * k;
+    qa[k] = 0.0;
+    pr[k] = i < m ? pv[i] : 0.0;
+  }
+  const double* A = L.A;
+  const double* D = L.D;
   for (int j = g; j < n; j += kDotGroups) {  // uniform within a group of 16 lanes
-    const double* c1 = L.A + static_cast<long long>(j) * m;
+    const double* c1 = A + static_cast<long long>(j) * m;
     double a1[kDotRows];
 #pragma unroll
     for (int k = 0; k < kDotRows; ++k) {
       const int i = r + kDotLanes * k;
       a1[k] = c1[i < m ? i : 0];
     }
-    const double dj = L.D[j];
+    const double dj = D[j];
     double s1 = 0.0;
 #pragma unroll
-    for (int k = 0; k < kDotRows; ++k) {
-      const int i = r + kDotLanes * k;
-      s1 = fma(a1[k], i < m ? pv[i] : 0.0, s1);
-    }
+    for (int k = 0; k < kDotRows; ++k) s1 = fma(a1[k], pr[k], s1);
     const double tj = group_sum<kDotLanes>(s1);
     if (atp && r == 0) atp[j] = tj;
     const double f = tj * dj;
