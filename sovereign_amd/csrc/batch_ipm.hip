@@ -139,43 +139,16 @@ struct Prec<double> {
   __device__ static __forceinline__ int row(int lk, int g) { return lk + 4 * g; }
 };
 
-// The two products below are bound by load round trips, not bytes: one
+// The products with A are bound by load round trips, not bytes: one
 // workgroup streams A (m x n FP64) through its CU, and a load of A takes
 // hundreds of cycles to return. Each thread therefore issues the loads of
 // a whole chunk (a column's rows, or a run of a row's columns) before it
-// uses any of them, so that one round trip serves many loads.
-
-// out_j = (A^T vec)_j for j < n, over the column-major A: 16 lanes per
-// column (rows r, r + 16, ...); vec (m) is staged in LDS first.
+// uses any of them, so that one round trip serves many loads. Over the
+// column-major A a group of 16 lanes takes a column (rows r, r + 16, ...):
+// the residual products and the normal-matrix product below.
 constexpr int kDotLanes = 16;
 constexpr int kDotGroups = kThreads / kDotLanes;
 constexpr int kDotRows = (kMaxRows + kDotLanes - 1) / kDotLanes;  // rows of a column per lane
-__device__ void gemv_t(const Lp& L, const double* vec, double* out, double* stage) {
-  const int tid = threadIdx.x;
-  __syncthreads();
-  for (int i = tid; i < L.m; i += kThreads) stage[i] = vec[i];
-  __syncthreads();
-  const int m = L.m, n = L.n;
-  const int g = tid / kDotLanes, r = tid % kDotLanes;
-  for (int j = g; j < n; j += kDotGroups) {  // uniform within a group of 16 lanes
-    const double* c1 = L.A + static_cast<long long>(j) * m;
-    double a1[kDotRows];
-#pragma unroll
-    for (int q = 0; q < kDotRows; ++q) {
-      const int i = r + kDotLanes * q;
-      a1[q] = c1[i < m ? i : 0];
-    }
-    double s1 = 0.0;
-#pragma unroll
-    for (int q = 0; q < kDotRows; ++q) {
-      const int i = r + kDotLanes * q;
-      s1 = fma(a1[q], i < m ? stage[i] : 0.0, s1);
-    }
-    s1 = group_sum<kDotLanes>(s1);
-    if (r == 0) out[j] = s1;
-  }
-  __syncthreads();
-}
 
 // out_i = base_i + (A vec)_i for i < m (base may be null), over the
 // row-major FP64 copy of A: 16 lanes per row (columns r, r + 16, ...),
@@ -1538,6 +1511,18 @@ T* dev_alloc(sov_batch* h, std::size_t count) {
   return static_cast<T*>(p);
 }
 
+// Row-major copies of the column-major m x n matrix a, leading dimension
+// ldr >= n, zero in columns n .. ldr - 1: FP64 and FP32 (rounded to nearest).
+__global__ void row_major_copies_kernel(int m, int n, int ldr, const double* __restrict__ a, float* __restrict__ a32,
+                                        double* __restrict__ a64r) {
+  const long long idx = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (idx >= static_cast<long long>(m) * ldr) return;
+  const int i = static_cast<int>(idx / ldr), j = static_cast<int>(idx % ldr);
+  const double v = j < n ? a[i + static_cast<long long>(j) * m] : 0.0;
+  a32[idx] = static_cast<float>(v);
+  a64r[idx] = v;
+}
+
 // The pair of timing events of one call, destroyed on every return path.
 struct EventPair {
   hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -1590,20 +1575,18 @@ int sov_batch_add_matrix(sov_batch* h, int32_t m, int32_t n, const double* a_col
   const std::size_t cnt = static_cast<std::size_t>(m) * n;
   const int ldr = row_ld(n);
   const std::size_t cntr = static_cast<std::size_t>(m) * ldr;
-  std::vector<float> af(cntr, 0.f);
-  for (int j = 0; j < n; ++j)
-    for (int i = 0; i < m; ++i)
-      af[static_cast<std::size_t>(i) * ldr + j] = static_cast<float>(a_colmajor[i + static_cast<std::size_t>(j) * m]);
-  std::vector<double> ar(cntr, 0.0);
-  for (int j = 0; j < n; ++j)
-    for (int i = 0; i < m; ++i) ar[static_cast<std::size_t>(i) * ldr + j] = a_colmajor[i + static_cast<std::size_t>(j) * m];
   double* a64 = dev_alloc<double>(h, cnt);
   float* a32 = dev_alloc<float>(h, cntr);
   double* a64r = dev_alloc<double>(h, cntr);
   if (!a64 || !a32 || !a64r) return SOV_ERR_NOMEM;
-  SOV_HIP_TRY(hipMemcpy(a64, a_colmajor, cnt * sizeof(double), hipMemcpyHostToDevice));
-  SOV_HIP_TRY(hipMemcpy(a32, af.data(), cntr * sizeof(float), hipMemcpyHostToDevice));
-  SOV_HIP_TRY(hipMemcpy(a64r, ar.data(), cntr * sizeof(double), hipMemcpyHostToDevice));
+  // the column-major copy goes up (the call returns after the caller's
+  // array has been read); the row-major copies are built from it on the
+  // device, ahead of any solve on the same stream
+  hipStream_t s = sov::stream();
+  SOV_HIP_TRY(hipMemcpyAsync(a64, a_colmajor, cnt * sizeof(double), hipMemcpyHostToDevice, s));
+  row_major_copies_kernel<<<static_cast<unsigned>((cntr + 255) / 256), 256, 0, s>>>(m, n, ldr, a64, a32, a64r);
+  SOV_HIP_TRY(hipGetLastError());
+  SOV_HIP_TRY(hipStreamSynchronize(s));
   h->mats.push_back({m, n, a64, a32, ldr, a64r});
   *id = static_cast<int32_t>(h->mats.size() - 1);
   return SOV_OK;

@@ -30,6 +30,8 @@
 #include <limits>
 #include <memory>
 #include <queue>
+#include <system_error>
+#include <thread>
 #include <vector>
 
 namespace {
@@ -238,14 +240,25 @@ class NodePresolve {
   // proves infeasible.
   template <typename Spec>
   void mark(std::vector<Spec>& kids) const {
-    if (kids.empty() || m_ == 0) return;
+    const std::vector<char> bad = infeasible(kids);
+    for (std::size_t k = 0; k < kids.size(); ++k)
+      if (bad[k]) kids[k].empty = true;
+  }
+
+  // The test of mark() without marking: entry k is 1 when child k's box is
+  // crossed or a row it touches cannot be met. Reads kids only.
+  template <typename Spec>
+  std::vector<char> infeasible(const std::vector<Spec>& kids) const {
+    std::vector<char> out(kids.size(), 0);
+    if (kids.empty() || m_ == 0) return out;
     const Spec& k0 = kids[0];
     std::vector<Activity> base(static_cast<std::size_t>(m_));
     for (int j = 0; j < n_; ++j)
       for (int p = cptr_[j]; p < cptr_[j + 1]; ++p) base[crow_[p]].add(cval_[p], k0.lb[j], k0.ub[j], 1.0);
     std::vector<Activity> act;
     std::vector<int> rows;
-    for (Spec& k : kids) {
+    for (std::size_t q = 0; q < kids.size(); ++q) {
+      const Spec& k = kids[q];
       if (k.empty) continue;
       bool bad = false;
       rows.clear();
@@ -265,8 +278,9 @@ class NodePresolve {
       else
         for (int r : rows)
           if (!bad) bad = violates(act[r], r);
-      if (bad) k.empty = true;
+      if (bad) out[q] = 1;
     }
+    return out;
   }
 
  private:
@@ -792,19 +806,37 @@ int sov_strong_branch(const sov_lp_model* node, const double* node_lb, const dou
   std::vector<ChildSpec> kids;
   children_specs(lb, ub, cand, x_node, F.n - F.nv, kids);
   const NodePresolve presolve(*node);
-  presolve.mark(kids);
   std::vector<ChildOut> res;
   int rc;
   if (child_solver == SOV_CHILD_BATCHED && batched_form_ok(node_lb, node->n_vars)) {
+    // The presolve test runs on a host thread while the GPU solves every
+    // child; a child it proves infeasible then gets the result it gets when
+    // marked before the solve (status 2), whatever its LP returned.
+    std::vector<char> pre_bad;
+    std::thread pre;
+    try {
+      pre = std::thread([&] { pre_bad = presolve.infeasible(kids); });
+    } catch (const std::system_error&) {
+      pre_bad = presolve.infeasible(kids);  // no thread to be had: the test runs here
+    }
     sov_batch* batch = nullptr;
-    if ((rc = sov_batch_create(&batch)) != SOV_OK) return rc;
+    rc = sov_batch_create(&batch);
     int32_t mid = 0;
-    rc = sov_batch_add_matrix(batch, F.m, F.n, F.A.data(), &mid);
+    if (rc == SOV_OK) rc = sov_batch_add_matrix(batch, F.m, F.n, F.A.data(), &mid);
     if (rc == SOV_OK) rc = solve_children(batch, mid, F, kids, *opt, res, kernel_ms, nullptr);
-    sov_batch_free(batch);
+    if (batch) sov_batch_free(batch);
+    if (pre.joinable()) pre.join();
+    if (rc == SOV_OK)
+      for (std::size_t k = 0; k < kids.size(); ++k)
+        if (pre_bad[k]) {
+          kids[k].empty = true;
+          res[k] = ChildOut{};
+          res[k].status = 2;
+        }
     if (rc == SOV_OK) rc = prove_infeasible(presolve, kids, res);
     if (rc == SOV_OK) rc = finish_unconverged(*node, kids, F, *opt, res, nullptr);
   } else {
+    presolve.mark(kids);
     rc = solve_children_one_by_one(*node, kids, F, *opt, res, nullptr);
     if (rc == SOV_OK) rc = prove_infeasible(presolve, kids, res);
   }
@@ -935,13 +967,31 @@ int sov_bnb_solve(const sov_lp_model* milp, const int8_t* is_int, const sov_bnb_
   warm.save_mu = bo->warm_mu > 0.0 ? bo->warm_mu : 0.1;
   long long lp_iterations = 0;
   auto solve_set = [&](std::vector<ChildSpec> kids, std::vector<ChildOut>& outv) {
-    presolve.mark(kids);
     int rc;
     if (batched) {
+      // presolve test on a host thread beside the GPU solve (as in
+      // sov_strong_branch); a child it proves infeasible gets status 2
+      std::vector<char> pre_bad;
+      std::thread pre;
+      try {
+        pre = std::thread([&] { pre_bad = presolve.infeasible(kids); });
+      } catch (const std::system_error&) {
+        pre_bad = presolve.infeasible(kids);
+      }
       rc = solve_children(batch.get(), mat_id, F, kids, *opt, outv, &kernel_ms, &lp_solves, warm);
+      if (pre.joinable()) pre.join();
+      if (rc == SOV_OK)
+        for (std::size_t k = 0; k < kids.size(); ++k)
+          if (pre_bad[k]) {
+            kids[k].empty = true;
+            outv[k] = ChildOut{};
+            outv[k].status = 2;
+            --lp_solves;  // settled by the presolve test, not by its LP
+          }
       if (rc == SOV_OK) rc = prove_infeasible(presolve, kids, outv);
       if (rc == SOV_OK) rc = finish_unconverged(*mdl, kids, F, *opt, outv, &lp_solves);
     } else {
+      presolve.mark(kids);
       rc = solve_children_one_by_one(*mdl, kids, F, *opt, outv, &lp_solves);
       if (rc == SOV_OK) rc = prove_infeasible(presolve, kids, outv);
     }
