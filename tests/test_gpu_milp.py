@@ -69,6 +69,29 @@ def test_strong_branch_children_match_oracle(gpu_api):
         assert rel(a["obj_down"], b["obj_down"]) <= 1e-8 and rel(a["obj_up"], b["obj_up"]) <= 1e-8
 
 
+def test_strong_branch_presolve_infeasible_child(gpu_api):
+    """A child that the node presolve proves infeasible (the row activity
+    test of presolve.cpp:186-246) is infeasible (status 2) on both paths,
+    the presolve running beside the batched solve: max x0 + x1 subject to
+    5 x0 + x1 <= 4, x binary; the LP optimum is x0 = 0.6, x1 = 1, and the up
+    child x0 = 1 has row activity at least 5 > 4. The down child is
+    x0 = 0, x1 = 1 (objective 1)."""
+    ptr, rows, vals = P.csc_from_columns([[0], [0]], [[5.0], [1.0]], 1)
+    model = P.LpModel(n_vars=2, n_rows=1, col_ptr=ptr, row_idx=rows, values=vals, obj=np.array([1.0, 1.0]),
+                      lb=np.zeros(2), ub=np.ones(2), row_sense=b"L", rhs=np.array([4.0]), maximize=True)
+    is_int = np.ones(2, dtype=np.int8)
+    root = O.lp_solve(model)
+    assert root["status"] == 0 and abs(root["primal"][0] - 0.6) < 1e-6
+    for solver in (MI.CHILD_BATCHED, MI.CHILD_PER_LP):
+        say(f"presolve-infeasible child, solver {solver}")
+        cands, best, _ = MI.strong_branch(model, is_int, root["primal"], root["objective"], max_candidates=4,
+                                          child_solver=solver)
+        assert [c["var"] for c in cands] == [0]
+        assert cands[0]["status_up"] == 2, cands[0]
+        assert cands[0]["status_down"] == 0 and rel(cands[0]["obj_down"], 1.0) <= 1e-8, cands[0]
+        assert best == 0
+
+
 def test_strong_branch_full_size_root(gpu_api):
     """The strong-branching workload at its real size: the root of the
     n=500, m=200 knapsack (seed 26119), 64 candidates x 2 = 128 child LPs of
