@@ -35,6 +35,9 @@ Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
                        [--paths mixed,fp64] [--no-branching] [--no-qp]
        python bench.py --only-qp [--steps K] [--no-cpu]   (the QP leg alone)
        python bench.py --only-branching   (the strong-branching leg alone)
+       python bench.py --sb-root-only   (the batched root strong-branching
+                                         call alone, warm-up plus three
+                                         timed calls: counter profiles)
        python bench.py --copy-probe   (a plain 256 MiB device copy, three
                                        times: the calibration run for the
                                        FETCH_SIZE / WRITE_SIZE counters)
@@ -388,6 +391,25 @@ def branching_leg(lib):
     return out
 
 
+def sb_root_calls(reps=3):
+    """The batched strong-branching call at the knapsack root (64 candidates,
+    128 child LPs) alone: one warm-up call, then `reps` timed calls (wall
+    seconds and the batched kernel's HIP-event milliseconds)."""
+    from sovereign_amd import api, milp, problems as P
+
+    model = P.multi_knapsack(500, 200, seed=P.BASE_SEED)
+    is_int = np.ones(model.n_vars, dtype=np.int8)
+    root = api.InteriorPointSolver(api.InteriorPointOptions(feasibility_tol=1e-9, optimality_tol=1e-9)).solve(model)
+    milp.strong_branch(model, is_int, root["primal"], root["objective"], max_candidates=8)
+    out = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        cb, _, kms = milp.strong_branch(model, is_int, root["primal"], root["objective"], max_candidates=64)
+        out.append({"wall_s": time.perf_counter() - t0, "kernel_ms": kms, "child_lps": 2 * len(cb)})
+        log(f"root strong branching: {2 * len(cb)} child LPs {out[-1]['wall_s'] * 1e3:.2f} ms wall, {kms:.2f} ms kernel")
+    return out
+
+
 def tree_leg(model, is_int, ranker, nodes=40, per_lp_nodes=4):
     """Wall seconds per branch-and-bound node over a tree (not only the
     root): the knapsack searched for a fixed node budget with each
@@ -506,6 +528,8 @@ def main():
     ap.add_argument("--no-qp", action="store_true")
     ap.add_argument("--only-qp", action="store_true", help="run the separable QP leg alone (profiling)")
     ap.add_argument("--only-branching", action="store_true", help="run the strong-branching leg alone (profiling)")
+    ap.add_argument("--sb-root-only", action="store_true",
+                    help="only the batched root strong-branching call, three timed calls (counter profiles)")
     ap.add_argument("--paths", default="mixed,fp64")
     ap.add_argument("--m", type=int, default=8192)
     ap.add_argument("--n", type=int, default=32768)
@@ -543,6 +567,9 @@ def main():
         from sovereign_amd import _capi
 
         print(json.dumps({"strong_branching": branching_leg(_capi.load())}), flush=True)
+        return
+    if args.sb_root_only:
+        print(json.dumps({"strong_branching_root": sb_root_calls()}), flush=True)
         return
 
     from sovereign_amd import _capi, api, problems as P
