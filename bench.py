@@ -97,10 +97,11 @@ def kernel_times(lib):
 
 
 def timed_solves(ipm, opts, steps, warmup, lib):
-    """Warmup, then `steps` timed solves with HIP events on the bulk SYRK only
-    (the roofline kernel; level 2 keeps event records off the factorization's
-    critical path), then one more solve with every kernel class timed, for
-    the breakdown. Returns (results, wall of the timed steps, SYRK times of
+    """Warmup, then `steps` timed solves with HIP events on one launch in
+    seven of the bulk SYRK and of the diagonal-block kernel (timing level 2:
+    their own launch events, kept off most of the factorization's critical
+    path), then one more solve with every kernel class timed, for the
+    breakdown. Returns (results, wall of the timed steps, SYRK times of
     the timed steps, per-class times of the extra solve)."""
     for _ in range(warmup):
         r = ipm.solve(opts)
@@ -192,13 +193,17 @@ def dominant_roofline(kt, traffic, shape_ids):
     live = {k: v for k, v in entries.items() if v[1] is not None}
     top = max(live, key=lambda k: live[k][0])
     total = sum(v[0] for v in live.values())
+    # one launch in `stride` of each kind carries events: the sampled time
+    # times the stride estimates the kind's time in the timed region
+    stride = kt[23] if kt[23] >= 1 else 1.0
     main = dict(live[top][1])
-    main["timed_region_ms"] = live[top][0]
+    main["timed_region_ms"] = live[top][0] * stride
+    main["launch_sampling"] = f"1 in {int(stride)} launches of each kind timed by its own launch events"
     others = []
     for k, (ms, e) in sorted(live.items(), key=lambda kv: -kv[1][0]):
         if k != top:
             e = dict(e)
-            e["timed_region_ms"] = ms
+            e["timed_region_ms"] = ms * stride
             others.append(e)
     main["share_of_event_timed_kernels"] = live[top][0] / total if total > 0 else None
     return main, others

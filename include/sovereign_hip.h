@@ -66,8 +66,10 @@ void* sov_stream(void);
  * kernels are bracketed by HIP events on the streams they run on (recording
  * does not wait; the totals are collected after one stream
  * synchronisation); 2 times only the kernels whose own launches record the
- * events (the bulk trailing-update SYRK and the diagonal-block kernel),
- * which keeps event traffic off the factorization's critical path.
+ * events (the bulk trailing-update SYRK and the diagonal-block kernel), one
+ * launch in out[23] of each, which keeps event traffic off the
+ * factorization's critical path (launch counts and times are those of the
+ * timed launches).
  * sov_kernel_times fills out[0..23] with the totals since the last call and
  * resets them:
  *   0 FP64 trailing-update SYRK ms (the bulk update of each panel, MFMA),
@@ -83,7 +85,9 @@ void* sov_stream(void);
  *   18 standalone SpMV kernel ms, 19 its launches, 20 its algorithmic bytes
  *      (12 nnz + 4 (rows + 1) + 8 (m + n) per product),
  *   21 triangular solve pairs, 22 their algorithmic bytes (the factor's
- *      lower triangle read twice, m (m + 1) elements per pair), 23 unused.
+ *      lower triangle read twice, m (m + 1) elements per pair),
+ *   23 the level-2 stride (one launch in this many of a kind is timed; 1
+ *      at the other levels).
  * No counterpart in the reference, which times nothing on the device. */
 void sov_set_kernel_timing(int32_t on);
 int sov_kernel_times(double* out24);
